@@ -1,0 +1,205 @@
+"""TEST INFRASTRUCTURE ONLY -- the oracle and the reference harness.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg may import this package, and only as the checker; the product path
+(``wannier_tools_amd`` and ``libwtsurf.so``) never does.
+
+* ``Oracle``      -- ctypes front of ``liboracle.so``, the from-scratch FP64 C
+                     restatement in ``wtsurf_oracle.c`` (functions and the
+                     reference lines they follow are listed in wtsurf_oracle.h).
+* ``RefHarness``  -- ctypes front of ``_ref/libwtref.so``: the reference's own
+                     Fortran (ham_qlayer2qlayer, surfgreen_1985, inv, mat_mul)
+                     compiled from its sources by ``oracle/Makefile``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libwtref.so")
+REFERENCE_SRC = "/root/reference/src"
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+def _d(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+def build(ref: bool = True) -> None:
+    """Compile liboracle.so, and _ref/libwtref.so when the reference tree exists."""
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+    if ref and os.path.isdir(REFERENCE_SRC):
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+class Oracle:
+    """FP64 C restatement of the hot path (column-major complex128 arrays)."""
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            build(ref=False)
+        lib = ctypes.CDLL(path)
+        lib.wto_ham_qlayer2qlayer.argtypes = [ctypes.c_int] * 4 + [_ip, _dp, _dp, _dp, _dp]
+        lib.wto_inv.argtypes = [ctypes.c_int, _dp]
+        lib.wto_mat_mul.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        lib.wto_mat_mul.restype = None
+        lib.wto_surfgreen_1985.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           _dp, _dp, _dp, _dp, _dp, ctypes.c_int,
+                                           ctypes.c_double, _ip]
+        lib.wto_spectral.argtypes = ([ctypes.c_int] * 4 + [_ip, _dp, ctypes.c_int, _dp,
+                                     ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, _ip,
+                                     ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _ip])
+        self.lib = lib
+
+    @staticmethod
+    def _model_arrays(irvec, hr):
+        irvec = np.ascontiguousarray(irvec, dtype=np.int32)
+        # per-R column-major blocks: hr[iR, m, n] -> memory [iR][n][m]
+        hr_cm = np.ascontiguousarray(np.asarray(hr, dtype=np.complex128).transpose(0, 2, 1))
+        return irvec, hr_cm
+
+    def ham_qlayer2qlayer(self, nw, np_, irvec, hr, k, ijmax=10):
+        irvec, hr_cm = self._model_arrays(irvec, hr)
+        n = nw * np_
+        h00 = np.zeros((n, n), dtype=np.complex128, order="F")
+        h01 = np.zeros((n, n), dtype=np.complex128, order="F")
+        kk = np.ascontiguousarray(k, dtype=np.float64)
+        rc = self.lib.wto_ham_qlayer2qlayer(nw, np_, irvec.shape[0], ijmax, _i(irvec),
+                                            _d(hr_cm.view(np.float64)), _d(kk),
+                                            h00.ctypes.data_as(_dp), h01.ctypes.data_as(_dp))
+        if rc != 0:
+            raise RuntimeError(f"wto_ham_qlayer2qlayer failed ({rc})")
+        return h00, h01
+
+    def inv(self, a):
+        a = np.array(a, dtype=np.complex128, order="F", copy=True)
+        info = self.lib.wto_inv(a.shape[0], a.ctypes.data_as(_dp))
+        return a, info
+
+    def mat_mul(self, a, b):
+        a = np.asfortranarray(a, dtype=np.complex128)
+        b = np.asfortranarray(b, dtype=np.complex128)
+        c = np.zeros_like(a, order="F")
+        self.lib.wto_mat_mul(a.shape[0], a.ctypes.data_as(_dp), b.ctypes.data_as(_dp),
+                             c.ctypes.data_as(_dp))
+        return c
+
+    def surfgreen_1985(self, omega, eta, h00, h01, itermax=100, accuracy=None):
+        from wannier_tools_amd.model import DECIMATION_ACCURACY
+        acc = DECIMATION_ACCURACY if accuracy is None else accuracy
+        h00 = np.asfortranarray(h00, dtype=np.complex128)
+        h01 = np.asfortranarray(h01, dtype=np.complex128)
+        n = h00.shape[0]
+        gll = np.zeros((n, n), dtype=np.complex128, order="F")
+        grr = np.zeros_like(gll)
+        gb = np.zeros_like(gll)
+        it = np.zeros(1, dtype=np.int32)
+        info = self.lib.wto_surfgreen_1985(n, omega, eta, h00.ctypes.data_as(_dp),
+                                           h01.ctypes.data_as(_dp), gll.ctypes.data_as(_dp),
+                                           grr.ctypes.data_as(_dp), gb.ctypes.data_as(_dp),
+                                           itermax, acc, _i(it))
+        return gll, grr, gb, int(it[0]), info
+
+    def spectral(self, nw, np_, irvec, hr, kpts, omegas, eta, top, bot, ijmax=10,
+                 itermax=100, accuracy=None, divisor=1.0, nthreads=0):
+        from wannier_tools_amd.model import DECIMATION_ACCURACY
+        acc = DECIMATION_ACCURACY if accuracy is None else accuracy
+        irvec, hr_cm = self._model_arrays(irvec, hr)
+        kpts = np.ascontiguousarray(kpts, dtype=np.float64).reshape(-1, 2)
+        omegas = np.ascontiguousarray(omegas, dtype=np.float64).ravel()
+        top = np.ascontiguousarray(top, dtype=np.int32)
+        bot = np.ascontiguousarray(bot, dtype=np.int32)
+        nk, nom = kpts.shape[0], omegas.shape[0]
+        dl = np.zeros((nk, nom), dtype=np.float64, order="F")
+        dr = np.zeros_like(dl)
+        db = np.zeros_like(dl)
+        its = np.zeros((nk, nom), dtype=np.int32, order="F")
+        rc = self.lib.wto_spectral(nw, np_, irvec.shape[0], ijmax, _i(irvec),
+                                   _d(hr_cm.view(np.float64)), nk, _d(kpts), nom, _d(omegas),
+                                   eta, top.size, _i(top), bot.size, _i(bot), itermax, acc,
+                                   divisor, nthreads, dl.ctypes.data_as(_dp),
+                                   dr.ctypes.data_as(_dp), db.ctypes.data_as(_dp),
+                                   its.ctypes.data_as(_ip))
+        if rc < 0:
+            raise RuntimeError(f"wto_spectral failed ({rc})")
+        return dl, dr, db, its
+
+
+class RefHarness:
+    """The reference's own Fortran routines (oracle/_ref/libwtref.so)."""
+
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        lib = ctypes.CDLL(path)
+        lib.wtref_set_model.argtypes = [ctypes.c_int] * 4 + [_ip, _dp]
+        lib.wtref_set_model.restype = None
+        lib.wtref_ham_qlayer2qlayer.argtypes = [_dp, _dp, _dp]
+        lib.wtref_ham_qlayer2qlayer.restype = None
+        lib.wtref_surfgreen_1985.argtypes = [ctypes.c_double, ctypes.c_double] + [_dp] * 5
+        lib.wtref_surfgreen_1985.restype = None
+        lib.wtref_inv.argtypes = [ctypes.c_int, _dp]
+        lib.wtref_inv.restype = None
+        lib.wtref_mat_mul.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        lib.wtref_mat_mul.restype = None
+        self.lib = lib
+        self.n = 0
+        self._keep = None
+
+    def set_model(self, nw, np_, irvec, hr, ijmax=10):
+        irvec = np.ascontiguousarray(irvec, dtype=np.int32)
+        hr_cm = np.ascontiguousarray(np.asarray(hr, dtype=np.complex128).transpose(0, 2, 1))
+        self._keep = (irvec, hr_cm)
+        self.lib.wtref_set_model(nw, np_, irvec.shape[0], ijmax, _i(irvec),
+                                 hr_cm.ctypes.data_as(_dp))
+        self.n = nw * np_
+
+    def ham_qlayer2qlayer(self, k):
+        n = self.n
+        h00 = np.zeros((n, n), dtype=np.complex128, order="F")
+        h01 = np.zeros((n, n), dtype=np.complex128, order="F")
+        kk = np.ascontiguousarray(k, dtype=np.float64)
+        self.lib.wtref_ham_qlayer2qlayer(_d(kk), h00.ctypes.data_as(_dp), h01.ctypes.data_as(_dp))
+        return h00, h01
+
+    def surfgreen_1985(self, omega, eta, h00, h01):
+        n = self.n
+        h00 = np.asfortranarray(h00, dtype=np.complex128)
+        h01 = np.asfortranarray(h01, dtype=np.complex128)
+        gll = np.zeros((n, n), dtype=np.complex128, order="F")
+        grr = np.zeros_like(gll)
+        gb = np.zeros_like(gll)
+        self.lib.wtref_surfgreen_1985(omega, eta, h00.ctypes.data_as(_dp), h01.ctypes.data_as(_dp),
+                                      gll.ctypes.data_as(_dp), grr.ctypes.data_as(_dp),
+                                      gb.ctypes.data_as(_dp))
+        return gll, grr, gb
+
+    def inv(self, a):
+        a = np.array(a, dtype=np.complex128, order="F", copy=True)
+        self.lib.wtref_inv(a.shape[0], a.ctypes.data_as(_dp))
+        return a
+
+    def mat_mul(self, a, b):
+        a = np.asfortranarray(a, dtype=np.complex128)
+        b = np.asfortranarray(b, dtype=np.complex128)
+        c = np.zeros_like(a, order="F")
+        self.lib.wtref_mat_mul(a.shape[0], a.ctypes.data_as(_dp), b.ctypes.data_as(_dp),
+                               c.ctypes.data_as(_dp))
+        return c
