@@ -1,0 +1,110 @@
+/*
+ * wtsurf -- MI355X (gfx950) surface Green's function library: C ABI.
+ *
+ * Drop-in boundary for the WannierTools surface-state hot path.  Every entry
+ * point takes plain pointers and sizes, returns 0 on success and a negative
+ * code on failure (wtsurf_last_error gives the reason), and is synchronous
+ * unless its name says otherwise.  One context per host thread.
+ *
+ * Array conventions are the reference's (Fortran): complex matrices are
+ * column-major interleaved (re, im) doubles with leading dimension = order;
+ * orbital lists are 1-based; grid outputs are laid out like the reference's
+ * dos_l(knv2, omeganum), k index fastest.  Energies are in whatever unit H(R)
+ * is given in (the reference works in Hartree).
+ */
+#ifndef WTSURF_H
+#define WTSURF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wtsurf_ctx wtsurf_ctx;
+
+/* Precision of the decimation. */
+enum {
+    WTSURF_FP64 = 0,        /* all arithmetic in FP64, as the reference */
+    WTSURF_FP32_REFINE = 1  /* FP32 MFMA decimation + FP64 refinement */
+};
+
+/* Per-instance status bits (wtsurf_grid_download / wtsurf_spectral_grid). */
+enum {
+    WTSURF_STATUS_SINGULAR = 1,      /* an exactly singular solve (identity used, as
+                                        src/inverse.f90:51-55 leaves it) */
+    WTSURF_STATUS_NOT_CONVERGED = 2  /* itermax reached (src/surfgreen.f90:86) */
+};
+
+/* Context on HIP device `device` with its own stream.  No reference
+ * counterpart (the reference has no device state). */
+int wtsurf_ctx_create(int device, wtsurf_ctx **ctx);
+void wtsurf_ctx_destroy(wtsurf_ctx *ctx);
+/* Message of the last failed call on ctx ("" if none; never NULL). */
+const char *wtsurf_last_error(const wtsurf_ctx *ctx);
+
+/* Model upload: the module-`para` state that get_hmnr_cell leaves for
+ * ham_qlayer2qlayer (src/readHmnR.f90:578-766, src/module.f90:957-968):
+ * Num_wann = nw, Np = np, nrpts_surfacecell = nr, ijmax, irvec_surfacecell
+ * as [nr][3] ints, HmnR_surfacecell as [nr] column-major nw x nw complex
+ * blocks already divided by ndegen. */
+int wtsurf_set_model(wtsurf_ctx *ctx, int nw, int np, int nr, int ijmax,
+                     const int *irvec, const double *hr);
+
+/* Decimation stop rule: the reference hard-codes itermax = 100 and
+ * accuracy = 1e-16 (a single-precision literal) on sum_ij |alpha_ij|
+ * (src/surfgreen.f90:39,42,120-122); those are the defaults. */
+int wtsurf_set_stop_rule(wtsurf_ctx *ctx, int itermax, double accuracy);
+
+/* TopOrbitals / BottomOrbitals (src/readinput.f90:2851-2962): top indices in
+ * 1..Ndim, bottom indices in 1..Num_wann (offset by Ndim - Num_wann as
+ * src/surfstat.f90:155).  Defaults: 1..Num_wann for both. */
+int wtsurf_set_orbitals(wtsurf_ctx *ctx, int ntop, const int *top, int nbot,
+                        const int *bot);
+
+/* ham_qlayer2qlayer(k, H00, H01) (src/ham_qlayer2qlayer.f90:1-102) for nk
+ * k points at once: kpts is [nk][2] (fractional, surface reciprocal
+ * basis); h00 and h01 receive [nk] column-major Ndim x Ndim blocks. */
+int wtsurf_ham_qlayer2qlayer(wtsurf_ctx *ctx, int nk, const double *kpts,
+                             double *h00, double *h01);
+
+/* surfgreen_1985(omega, GLL, GRR, GB, H00, H01, ones, eta)
+ * (src/surfgreen.f90:7-137) for one instance of order n: full GLL, GRR, GB
+ * out.  iters (may be NULL) receives the decimation iteration count.  Does
+ * not need a model. */
+int wtsurf_surfgreen_1985(wtsurf_ctx *ctx, int n, double omega, double eta,
+                          const double *h00, const double *h01, double *gll,
+                          double *grr, double *gb, int *iters);
+
+/* The (k, omega) loops of surfstat (src/surfstat.f90:123-160; divisor = 1)
+ * and SurfaceDOSkk (src/fermiarc.f90:206-248; divisor = pi): for every k
+ * in kpts and every omega, decimate and take the traces
+ *   dos_l    = -sum_{io in top} Im GLL(io,io) / divisor
+ *   dos_r    = -sum_{b in bot}  Im GRR(Ndim-Num_wann+b, same) / divisor
+ *   dos_bulk = -sum_{i=1..Ndim} Im GB(i,i) / divisor
+ * Outputs are [nomega][nk] (k fastest).  iters and status may be NULL. */
+int wtsurf_spectral_grid(wtsurf_ctx *ctx, int nk, const double *kpts,
+                         int nomega, const double *omega, double eta,
+                         double divisor, int precision, double *dos_l,
+                         double *dos_r, double *dos_bulk, int *iters,
+                         int *status);
+
+/* The same computation split so inputs can stay resident in device memory:
+ * upload the grid once, run (enqueued on the context stream; returns after
+ * the launches are queued), wait, fetch.  wtsurf_grid_timing reports the
+ * device time of the last run's Fourier and decimation stages (HIP events
+ * on the context stream).  No reference counterpart. */
+int wtsurf_grid_upload(wtsurf_ctx *ctx, int nk, const double *kpts, int nomega,
+                       const double *omega);
+int wtsurf_grid_run_async(wtsurf_ctx *ctx, double eta, double divisor,
+                          int precision);
+int wtsurf_sync(wtsurf_ctx *ctx);
+int wtsurf_grid_timing(wtsurf_ctx *ctx, float *ms_fourier, float *ms_decimate);
+int wtsurf_grid_download(wtsurf_ctx *ctx, double *dos_l, double *dos_r,
+                         double *dos_bulk, int *iters, int *status);
+
+/* Largest Ndim the FP64 decimation kernels are built for. */
+int wtsurf_max_ndim(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WTSURF_H */

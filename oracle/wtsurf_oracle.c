@@ -97,7 +97,13 @@ static int lu_inverse(int n, cplx *a, cplx *b, int *ipiv)
             for (int i = j + 1; i < n; ++i) col[i] -= l[i] * u;
         }
     }
-    if (info) return info;
+    if (info) {
+        /* zgesv skips the solve on a zero pivot and inv copies back its
+         * identity right-hand side (src/inverse.f90:44-55). */
+        memset(a, 0, sizeof(cplx) * (size_t)n * n);
+        for (int i = 0; i < n; ++i) a[i + (size_t)n * i] = 1.0;
+        return info;
+    }
     /* B = P * I */
     memset(b, 0, sizeof(cplx) * (size_t)n * n);
     for (int i = 0; i < n; ++i) b[i + (size_t)n * i] = 1.0;

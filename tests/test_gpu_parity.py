@@ -1,0 +1,176 @@
+"""Parity of the HIP path (through the C ABI) with the reference and the oracle.
+
+Bar: the Fourier sum to 1e-14 relative (device sincos vs libm), the FP64
+decimation to 1e-10 relative on the spectral traces A(k, omega) (dos_l, dos_r,
+dos_bulk) and on single diagonal entries of G within the FP64
+reproducibility floor measured between the oracle and the reference
+(5e-10 at Ndim = 72, eta = 1e-3 eV).
+"""
+import numpy as np
+import pytest
+
+from helpers import chain_bulk_g, chain_surface_g, golden_model, max_rel
+from wannier_tools_amd import model as M
+from wannier_tools_amd.surface import STATUS_NOT_CONVERGED, STATUS_SINGULAR, WtsurfError
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["ndim16", "np3_ndim12", "ndim72"]
+TOL_DIAG = {"ndim16": 1e-11, "np3_ndim12": 1e-11, "ndim72": 5e-10}
+TOL_TRACE = 1e-10
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fourier_vs_reference_golden(gpu, golden, name):
+    _say("fourier", name)
+    g = golden(name)
+    nw, np_, irvec, hr = golden_model(g, name)
+    gpu.set_model_arrays(nw, np_, irvec, hr)
+    h00, h01 = gpu.ham_qlayer2qlayer(g["kpts"])
+    scale = np.max(np.abs(g["h00"]))
+    assert np.max(np.abs(h00 - g["h00"])) < 1e-14 * scale
+    assert np.max(np.abs(h01 - g["h01"])) < 1e-14 * scale
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_surfgreen_1985_vs_reference_golden(gpu, golden, name):
+    _say("surfgreen_1985", name)
+    g = golden(name)
+    for ik in range(g["kpts"].shape[0]):
+        for iw, w in enumerate(g["omegas"]):
+            gll, grr, gb, it = gpu.surfgreen_1985(w, g["h00"][ik], g["h01"][ik], float(g["eta"]))
+            assert 1 <= it <= 100
+            tol = TOL_DIAG[name]
+            assert max_rel(np.diag(gll), g["diag_gll"][ik, iw]) < tol
+            assert max_rel(np.diag(grr), g["diag_grr"][ik, iw]) < tol
+            assert max_rel(np.diag(gb), g["diag_gb"][ik, iw]) < tol
+    if "full_g" in g:
+        gll, grr, gb, _ = gpu.surfgreen_1985(g["omegas"][0], g["h00"][0], g["h01"][0], float(g["eta"]))
+        for mine, ref in zip((gll, grr, gb), g["full_g"]):
+            assert np.max(np.abs(mine - ref)) < 1e-11 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_spectral_grid_vs_reference_golden(gpu, golden, oracle_lib, name):
+    _say("spectral_grid", name)
+    g = golden(name)
+    nw, np_, irvec, hr = golden_model(g, name)
+    gpu.set_model_arrays(nw, np_, irvec, hr)
+    dl, dr, db, its, st = gpu.spectral_grid(g["kpts"], g["omegas"], float(g["eta"]))
+    assert max_rel(dl, g["dos_l"]) < TOL_TRACE
+    assert max_rel(dr, g["dos_r"]) < TOL_TRACE
+    assert max_rel(db, g["dos_bulk"]) < TOL_TRACE
+    assert np.all(st == 0)
+    top = M.default_top_orbitals(nw)
+    *_, oits = oracle_lib.spectral(nw, np_, irvec, hr, g["kpts"], g["omegas"], float(g["eta"]), top, top)
+    assert np.array_equal(its, oits)
+
+
+def test_spectral_grid_ndim16_subgrid_vs_oracle(gpu, oracle_lib):
+    """A 21 k x 41 omega slice of the Ndim = 16 k-path workload."""
+    _say("subgrid ndim16")
+    mdl = M.model_ndim16()
+    irvec, hr = mdl.surface_cell()
+    gpu.set_model_arrays(8, 2, irvec, hr)
+    kp = M.kpath_slab([((-0.5, 0.0), (0.5, 0.0))], 201)[::10]
+    om = M.omega_grid(-1.0, 1.0, 401)[::10] * M.EV2HARTREE
+    eta = 1e-3 * M.EV2HARTREE
+    dl, dr, db, its, st = gpu.spectral_grid(kp, om, eta)
+    top = M.default_top_orbitals(8)
+    odl, odr, odb, oits = oracle_lib.spectral(8, 2, irvec, hr, kp, om, eta, top, top)
+    assert max_rel(dl, odl) < TOL_TRACE and max_rel(dr, odr) < TOL_TRACE and max_rel(db, odb) < TOL_TRACE
+    assert np.mean(its == oits) > 0.99
+
+
+@pytest.mark.parametrize("nw,np_", [(1, 1), (3, 1), (5, 3), (12, 3), (40, 2)])
+def test_uneven_orders_vs_oracle(gpu, oracle_lib, nw, np_):
+    """Orders that are not multiples of the 16-wide MFMA tile, custom orbital lists,
+    and the SurfaceDOSkk trace divisor pi."""
+    _say("orders", nw, np_)
+    mdl = M.synthetic_model(nw, 2, 0.4, M.decay_l2(1.5), seed=100 + nw)
+    irvec, hr = mdl.surface_cell()
+    gpu.set_model_arrays(nw, np_, irvec, hr)
+    n = nw * np_
+    top = np.array(sorted({1, n, (n + 1) // 2}), dtype=np.int32)
+    bot = np.array(sorted({1, nw}), dtype=np.int32)
+    gpu.set_orbitals(top, bot)
+    rng = np.random.default_rng(nw)
+    kp = rng.uniform(-0.5, 0.5, (3, 2))
+    om = rng.uniform(-1, 1, 4) * M.EV2HARTREE
+    eta = 4e-3 * M.EV2HARTREE
+    dl, dr, db, its, st = gpu.spectral_grid(kp, om, eta, divisor=M.REF_PI)
+    odl, odr, odb, oits = oracle_lib.spectral(nw, np_, irvec, hr, kp, om, eta, top, bot,
+                                              divisor=M.REF_PI)
+    gpu.set_orbitals(M.default_top_orbitals(nw), M.default_top_orbitals(nw))
+    assert max_rel(dl, odl) < TOL_TRACE and max_rel(dr, odr) < TOL_TRACE and max_rel(db, odb) < TOL_TRACE
+
+
+def test_chain_closed_form_on_device(gpu):
+    _say("chain")
+    e0, t = 0.2 * M.EV2HARTREE, -0.8 * M.EV2HARTREE
+    for w_ev in (-1.7, 0.0, 2.5):
+        z = w_ev * M.EV2HARTREE + 2e-3j * M.EV2HARTREE
+        gll, grr, gb, it = gpu.surfgreen_1985(z.real, np.array([[e0]]), np.array([[t]]), z.imag)
+        gs = chain_surface_g(z, e0, t)
+        assert abs(gll[0, 0] - gs) < 1e-9 * abs(gs) and abs(grr[0, 0] - gs) < 1e-9 * abs(gs)
+        gbk = chain_bulk_g(z, e0, t)
+        assert abs(gb[0, 0] - gbk) < 1e-9 * abs(gbk)
+
+
+def test_block_diagonal_chains_on_device(gpu):
+    """72 independent chains (diagonal H00, H01): exact answers at the headline order."""
+    _say("diag chains")
+    n = 72
+    rng = np.random.default_rng(2)
+    e0 = rng.uniform(-0.5, 0.5, n) * M.EV2HARTREE
+    t = rng.uniform(0.3, 0.9, n) * M.EV2HARTREE * rng.choice([-1, 1], n)
+    w, eta = 0.17 * M.EV2HARTREE, 1e-3 * M.EV2HARTREE
+    gll, grr, gb, it = gpu.surfgreen_1985(w, np.diag(e0).astype(complex), np.diag(t).astype(complex), eta)
+    z = w + 1j * eta
+    for i in range(n):
+        gs = chain_surface_g(z, e0[i], t[i])
+        assert abs(gll[i, i] - gs) < 1e-8 * abs(gs)
+        gbk = chain_bulk_g(z, e0[i], t[i])
+        assert abs(gb[i, i] - gbk) < 1e-8 * abs(gbk)
+
+
+def test_status_singular_and_not_converged(gpu, oracle_lib):
+    """eta = 0, H01 = 0 and omega on an eigenvalue of H00: exactly singular
+    solve -> identity result and the singular bit.  itermax = 2 on an in-band
+    energy -> the not-converged bit."""
+    _say("status")
+    h00 = np.diag([0.25, -0.5]).astype(complex)
+    h01 = np.zeros((2, 2), complex)
+    gll, grr, gb, it = gpu.surfgreen_1985(0.25, h00, h01, 0.0)
+    assert np.array_equal(gb, np.eye(2))
+    mdl = M.model_ndim16()
+    irvec, hr = mdl.surface_cell()
+    gpu.set_model_arrays(8, 2, irvec, hr)
+    gpu.set_stop_rule(2, M.DECIMATION_ACCURACY)
+    try:
+        dl, dr, db, its, st = gpu.spectral_grid([[0.1, 0.0]], [0.01], 1e-5)
+    finally:
+        gpu.set_stop_rule()
+    assert its[0, 0] == 2 and (st[0, 0] & STATUS_NOT_CONVERGED)
+
+
+def test_invalid_arguments_raise(gpu):
+    _say("invalid")
+    mdl = M.model_ndim16()
+    irvec, hr = mdl.surface_cell()
+    gpu.set_model_arrays(8, 2, irvec, hr)
+    with pytest.raises(WtsurfError):
+        gpu.set_orbitals([0], [1])
+    gpu.set_orbitals(M.default_top_orbitals(8), M.default_top_orbitals(8))
+    with pytest.raises(WtsurfError):
+        gpu.set_orbitals([1], [9])  # bottom index beyond Num_wann
+    gpu.set_orbitals(M.default_top_orbitals(8), M.default_top_orbitals(8))
+    with pytest.raises(WtsurfError):
+        gpu.set_stop_rule(0, 1e-16)
+    big = np.zeros((400, 400), complex)
+    with pytest.raises(WtsurfError):
+        gpu.surfgreen_1985(0.0, big, big, 1e-3)

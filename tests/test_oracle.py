@@ -1,0 +1,227 @@
+"""Pins the oracle (oracle/wtsurf_oracle.c) before anything is checked against it.
+
+Each oracle function is held against (i) the reference's own outputs stored
+in tests/golden/ (written by tests/golden/make_golden.py from the reference
+Fortran), (ii) the reference compiled in oracle/_ref where present, and
+(iii) what the mathematics fixes: the semi-infinite chain closed form,
+block-diagonal chains, a finite-slab brute force, unitary invariance of the
+bulk trace and numpy's inv / matmul.  A dropped term, a wrong sign, a swapped
+index or a transposed operand in the oracle fails at least one of these.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import chain_bulk_g, chain_surface_g, golden_model, max_rel
+from wannier_tools_amd import model as M
+
+CASES = ["ndim16", "np3_ndim12", "ndim72"]
+# Two FP64 implementations with different LU / GEMM summation orders agree to
+# about cond * 1e-16.  At Ndim = 72, eta = 1e-3 eV the oracle and the
+# reference (OpenBLAS zgesv / zgemm) differ by up to 1.4e-10 on single
+# diagonal entries and 4e-11 on the traces; the tolerances sit above that floor.
+TOL_DIAG = {"ndim16": 1e-11, "np3_ndim12": 1e-11, "ndim72": 5e-10}
+TOL_TRACE = {"ndim16": 1e-11, "np3_ndim12": 1e-11, "ndim72": 1e-10}
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_fourier_matches_reference_golden(oracle_lib, golden, name):
+    g = golden(name)
+    nw, np_, irvec, hr = golden_model(g, name)
+    for ik, k in enumerate(g["kpts"]):
+        h00, h01 = oracle_lib.ham_qlayer2qlayer(nw, np_, irvec, hr, k)
+        # same operations in the same order as the reference: bit-for-bit
+        assert np.array_equal(h00, g["h00"][ik])
+        assert np.array_equal(h01, g["h01"][ik])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_decimation_matches_reference_golden(oracle_lib, golden, name):
+    g = golden(name)
+    for ik in range(g["kpts"].shape[0]):
+        for iw, w in enumerate(g["omegas"]):
+            gll, grr, gb, it, info = oracle_lib.surfgreen_1985(w, float(g["eta"]), g["h00"][ik],
+                                                                g["h01"][ik])
+            assert info == 0 and 1 <= it <= 100
+            tol = TOL_DIAG[name]
+            assert max_rel(np.diag(gll), g["diag_gll"][ik, iw]) < tol
+            assert max_rel(np.diag(grr), g["diag_grr"][ik, iw]) < tol
+            assert max_rel(np.diag(gb), g["diag_gb"][ik, iw]) < tol
+    if "full_g" in g:
+        gll, grr, gb, _, _ = oracle_lib.surfgreen_1985(g["omegas"][0], float(g["eta"]),
+                                                        g["h00"][0], g["h01"][0])
+        for mine, ref in zip((gll, grr, gb), g["full_g"]):
+            assert np.max(np.abs(mine - ref)) < 1e-11 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_spectral_traces_match_reference_golden(oracle_lib, golden, name):
+    g = golden(name)
+    nw, np_, irvec, hr = golden_model(g, name)
+    top = M.default_top_orbitals(nw)
+    dl, dr, db, its = oracle_lib.spectral(nw, np_, irvec, hr, g["kpts"], g["omegas"],
+                                          float(g["eta"]), top, top)
+    assert max_rel(dl, g["dos_l"]) < TOL_TRACE[name]
+    assert max_rel(dr, g["dos_r"]) < TOL_TRACE[name]
+    assert max_rel(db, g["dos_bulk"]) < TOL_TRACE[name]
+    assert np.all(its >= 1)
+
+
+def test_fourier_single_r_closed_form(oracle_lib):
+    """One R = (2, -1, 1) hopping: H01 block (1,1) for Np = 1 is
+    H(R) e^{i 2 pi (2 k1 - k2)} with the reference's truncated pi."""
+    rng = np.random.default_rng(3)
+    h = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+    irvec = np.array([[2, -1, 1]], dtype=np.int32)
+    k = np.array([0.137, -0.291])
+    h00, h01 = oracle_lib.ham_qlayer2qlayer(3, 1, irvec, h[None], k)
+    ph = 2.0 * M.REF_PI * (2 * k[0] - k[1])
+    assert np.allclose(h01, h * (np.cos(ph) + 1j * np.sin(ph)), rtol=0, atol=1e-15)
+    assert np.all(h00 == 0)
+
+
+def test_fourier_np2_block_placement(oracle_lib):
+    """Np = 2: H00 = [[H0, H1], [H1^+, H0]], H01 = [[H2, H3], [H1, H2]] where
+    Hd sums the R with R3 = d (k = 0 makes every phase 1)."""
+    nw = 2
+    rng = np.random.default_rng(5)
+    blocks = {d: rng.normal(size=(nw, nw)) + 1j * rng.normal(size=(nw, nw)) for d in range(-3, 4)}
+    irvec = np.array([[0, 0, d] for d in range(-3, 4)], dtype=np.int32)
+    hr = np.stack([blocks[d] for d in range(-3, 4)])
+    h00, h01 = oracle_lib.ham_qlayer2qlayer(nw, 2, irvec, hr, [0.0, 0.0])
+    z = np.zeros((nw, nw))
+    e00 = np.block([[blocks[0], blocks[1]], [blocks[-1], blocks[0]]])
+    e01 = np.block([[blocks[2], blocks[3]], [blocks[1], blocks[2]]])
+    assert np.array_equal(h00, e00)
+    assert np.array_equal(h01, e01)
+    # ijmax clips |R3|: with ijmax = 1 the R3 = 2, 3 blocks vanish
+    h00, h01 = oracle_lib.ham_qlayer2qlayer(nw, 2, irvec, hr, [0.0, 0.0], ijmax=1)
+    assert np.array_equal(h01, np.block([[z, z], [blocks[1], z]]))
+
+
+def test_inverse_and_product_against_numpy(oracle_lib):
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 7, 16, 33):
+        a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        b = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        inv, info = oracle_lib.inv(a)
+        assert info == 0
+        assert np.max(np.abs(inv - np.linalg.inv(a))) < 1e-12 * np.max(np.abs(inv))
+        assert np.max(np.abs(oracle_lib.mat_mul(a, b) - a @ b)) < 1e-13 * n
+    # exactly singular: zgesv info = index of the zero pivot, and inv hands
+    # back the unsolved identity right-hand side
+    s = np.zeros((3, 3), dtype=np.complex128)
+    s[0, 0] = 1
+    inv, info = oracle_lib.inv(s)
+    assert info == 2 and np.array_equal(inv, np.eye(3))
+
+
+@pytest.mark.parametrize("w_ev", [-1.7, -0.3, 0.0, 0.9, 2.5])
+def test_chain_closed_form(oracle_lib, w_ev):
+    """1-orbital chain: GLL = GRR = surface g, GB = bulk g, both in closed form."""
+    e0, t, tpar, k1 = 0.2, -0.8, 0.1, 0.13
+    mdl = M.chain_model(e0, t, tpar)
+    irvec, hr = mdl.surface_cell()
+    h00, h01 = oracle_lib.ham_qlayer2qlayer(1, 1, irvec, hr, [k1, 0.0])
+    eps = (e0 + 2 * tpar * np.cos(2 * M.REF_PI * k1)) * M.EV2HARTREE
+    assert abs(h00[0, 0] - eps) < 1e-15 and abs(h01[0, 0] - t * M.EV2HARTREE) < 1e-16
+    w, eta = w_ev * M.EV2HARTREE, 2e-3 * M.EV2HARTREE
+    gll, grr, gb, it, info = oracle_lib.surfgreen_1985(w, eta, h00, h01)
+    z = w + 1j * eta
+    gs = chain_surface_g(z, eps, t * M.EV2HARTREE)
+    assert abs(gll[0, 0] - gs) < 1e-9 * abs(gs)
+    assert abs(grr[0, 0] - gs) < 1e-9 * abs(gs)
+    gbulk = chain_bulk_g(z, eps, t * M.EV2HARTREE)
+    assert abs(gb[0, 0] - gbulk) < 1e-9 * abs(gbulk)
+
+
+def test_block_diagonal_chains(oracle_lib):
+    """A diagonal model is N independent chains; every diagonal entry has its
+    own closed form (exercises the order-N path with exact answers)."""
+    n = 9
+    rng = np.random.default_rng(2)
+    e0 = rng.uniform(-0.5, 0.5, n) * M.EV2HARTREE
+    t = rng.uniform(0.3, 0.9, n) * M.EV2HARTREE * rng.choice([-1, 1], n)
+    h00 = np.diag(e0).astype(np.complex128)
+    h01 = np.diag(t).astype(np.complex128)
+    w, eta = 0.17 * M.EV2HARTREE, 1e-3 * M.EV2HARTREE
+    gll, grr, gb, it, info = oracle_lib.surfgreen_1985(w, eta, h00, h01)
+    z = w + 1j * eta
+    for i in range(n):
+        gs = chain_surface_g(z, e0[i], t[i])
+        assert abs(gll[i, i] - gs) < 1e-8 * abs(gs)
+        gbk = chain_bulk_g(z, e0[i], t[i])
+        assert abs(gb[i, i] - gbk) < 1e-8 * abs(gbk)
+    assert np.max(np.abs(gll - np.diag(np.diag(gll)))) == 0.0
+
+
+def test_finite_slab_brute_force(oracle_lib):
+    """Top-layer block of a long finite slab, inverted directly, converges to
+    GLL as e^{-eta L / v}; at eta = 0.05 eV and L = 400 layers it is exact to
+    far below the tolerance.  GRR is the bottom-layer block."""
+    mdl = M.synthetic_model(3, 1, 0.3, M.decay_l1, seed=21)
+    irvec, hr = mdl.surface_cell()
+    h00, h01 = oracle_lib.ham_qlayer2qlayer(3, 1, irvec, hr, [0.21, -0.08])
+    n, L = 3, 400
+    w, eta = 0.05 * M.EV2HARTREE, 0.05 * M.EV2HARTREE
+    hs = np.zeros((n * L, n * L), dtype=np.complex128)
+    for l in range(L):
+        hs[l * n:(l + 1) * n, l * n:(l + 1) * n] = h00
+        if l + 1 < L:
+            hs[l * n:(l + 1) * n, (l + 1) * n:(l + 2) * n] = h01
+            hs[(l + 1) * n:(l + 2) * n, l * n:(l + 1) * n] = h01.conj().T
+    gs = np.linalg.inv((w + 1j * eta) * np.eye(n * L) - hs)
+    gll, grr, gb, it, info = oracle_lib.surfgreen_1985(w, eta, h00, h01)
+    assert np.max(np.abs(gll - gs[:n, :n])) < 1e-8 * np.max(np.abs(gll))
+    assert np.max(np.abs(grr - gs[-n:, -n:])) < 1e-8 * np.max(np.abs(grr))
+    mid = (L // 2) * n
+    assert np.max(np.abs(gb - gs[mid:mid + n, mid:mid + n])) < 1e-8 * np.max(np.abs(gb))
+
+
+def test_bulk_trace_unitary_invariance(oracle_lib):
+    """Tr GB is invariant under H(R) -> U H(R) U^+ with one U for every R."""
+    mdl = M.synthetic_model(4, 1, 0.3, M.decay_l1, seed=9)
+    irvec, hr = mdl.surface_cell()
+    rng = np.random.default_rng(4)
+    u, _ = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))
+    hr_u = np.einsum("ij,rjk,lk->ril", u, hr, u.conj())
+    w, eta = -0.1 * M.EV2HARTREE, 3e-3 * M.EV2HARTREE
+    tr = []
+    for h in (hr, hr_u):
+        h00, h01 = oracle_lib.ham_qlayer2qlayer(4, 2, irvec, h, [0.3, 0.1])
+        gll, grr, gb, _, _ = oracle_lib.surfgreen_1985(w, eta, h00, h01)
+        tr.append((np.trace(gb), np.trace(gll)))
+    assert abs(tr[0][0] - tr[1][0]) < 1e-9 * abs(tr[0][0])
+    assert abs(tr[0][1] - tr[1][1]) < 1e-9 * abs(tr[0][1])
+
+
+def test_spectral_function_is_non_negative(oracle_lib):
+    mdl = M.model_ndim16()
+    irvec, hr = mdl.surface_cell()
+    kp = M.kpath_slab([((-0.5, 0.0), (0.5, 0.0))], 9)
+    om = M.omega_grid(-1.0, 1.0, 11) * M.EV2HARTREE
+    top = M.default_top_orbitals(8)
+    dl, dr, db, its = oracle_lib.spectral(8, 2, irvec, hr, kp, om, 1e-3 * M.EV2HARTREE, top, top)
+    assert np.all(dl > 0) and np.all(dr > 0) and np.all(db > 0)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built (no reference tree)")
+def test_oracle_against_compiled_reference_random_instances(oracle_lib):
+    ref = oracle.RefHarness()
+    mdl = M.synthetic_model(5, 2, 0.4, M.decay_l2(1.5), seed=31)
+    irvec, hr = mdl.surface_cell()
+    ref.set_model(5, 3, irvec, hr, ijmax=10)
+    rng = np.random.default_rng(8)
+    for _ in range(4):
+        k = rng.uniform(-0.5, 0.5, 2)
+        a, b = ref.ham_qlayer2qlayer(k)
+        a2, b2 = oracle_lib.ham_qlayer2qlayer(5, 3, irvec, hr, k)
+        assert np.array_equal(a, a2) and np.array_equal(b, b2)
+        w = rng.uniform(-1, 1) * M.EV2HARTREE
+        r = ref.surfgreen_1985(w, 2e-3 * M.EV2HARTREE, a, b)
+        o = oracle_lib.surfgreen_1985(w, 2e-3 * M.EV2HARTREE, a, b)
+        for x, y in zip(o[:3], r):
+            assert np.max(np.abs(x - y)) < 1e-11 * np.max(np.abs(y))
+        m = rng.normal(size=(15, 15)) + 1j * rng.normal(size=(15, 15))
+        assert np.max(np.abs(ref.inv(m) - oracle_lib.inv(m)[0])) < 1e-12 * np.max(np.abs(ref.inv(m)))
+        assert np.max(np.abs(ref.mat_mul(m, m.T) - oracle_lib.mat_mul(m, m.T))) < 1e-12

@@ -1,0 +1,448 @@
+// C-ABI layer of libwtsurf (declared in include/wtsurf.h): argument checks,
+// device buffers owned by the context, launches on the context stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/wtsurf.h"
+#include "wtsurf_internal.h"
+
+using namespace wtsurf;
+
+namespace {
+
+// Decimation stop rule of src/surfgreen.f90:39,42 (1e-16 as a real(4) literal).
+constexpr int kRefItermax = 100;
+constexpr double kRefAccuracy = (double)1e-16f;
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t want) {
+        if (want <= n && p) return 0;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (want == 0) return 0;
+        if (hipMalloc(&p, want * sizeof(T)) != hipSuccess) return -1;
+        n = want;
+        return 0;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct wtsurf_ctx {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::string err;
+
+    // model
+    bool has_model = false;
+    int nw = 0, np = 0, nr = 0, ijmax = 0, n = 0;
+    int nslot = 0, d_lo = 0;
+    DevBuf<double2> hr;
+    DevBuf<int> irvec, slot_begin;
+    // stop rule and orbitals
+    int itermax = kRefItermax;
+    double accuracy = kRefAccuracy;
+    std::vector<int> top, bot;
+    bool orbitals_set = false;
+    DevBuf<int> d_top, d_bot;
+    // grid
+    int nk = 0, nomega = 0;
+    DevBuf<double> kpts, omega;
+    DevBuf<double2> phase, hk;
+    DevBuf<double> dos_l, dos_r, dos_b;
+    DevBuf<int> iters, status;
+    // decimation workspace
+    DevBuf<double2> work;
+    DevBuf<unsigned> queue;
+    int work_q = 0, work_slots = 0;
+    bool ran = false;
+
+    int fail(const std::string &m) {
+        err = m;
+        return -1;
+    }
+    int hipfail(hipError_t e, const char *what) {
+        err = std::string(what) + ": " + hipGetErrorString(e);
+        return -2;
+    }
+};
+
+#define WT_HIP(ctx, call)                                   \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return (ctx)->hipfail(e_, #call); \
+    } while (0)
+
+static int use_device(wtsurf_ctx *c) {
+    WT_HIP(c, hipSetDevice(c->device));
+    return 0;
+}
+
+// Upload the orbital lists (default 1..Num_wann for both, as
+// src/readinput.f90:2857,2920-2935 for SOC = 0 single-cell defaults).
+static int sync_orbitals(wtsurf_ctx *c) {
+    if (!c->orbitals_set) {
+        c->top.resize(c->nw);
+        c->bot.resize(c->nw);
+        std::iota(c->top.begin(), c->top.end(), 1);
+        std::iota(c->bot.begin(), c->bot.end(), 1);
+    }
+    for (int v : c->top)
+        if (v < 1 || v > c->n) return c->fail("top orbital index outside 1..Ndim");
+    for (int v : c->bot)
+        if (v < 1 || v > c->nw) return c->fail("bottom orbital index outside 1..Num_wann");
+    if (c->d_top.ensure(std::max<size_t>(c->top.size(), 1)) ||
+        c->d_bot.ensure(std::max<size_t>(c->bot.size(), 1)))
+        return c->fail("out of device memory (orbital lists)");
+    if (!c->top.empty())
+        WT_HIP(c, hipMemcpy(c->d_top.p, c->top.data(), c->top.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    if (!c->bot.empty())
+        WT_HIP(c, hipMemcpy(c->d_bot.p, c->bot.data(), c->bot.size() * sizeof(int),
+                            hipMemcpyHostToDevice));
+    return 0;
+}
+
+static int ensure_work(wtsurf_ctx *c, int q) {
+    const int slots = decimate_f64_slots(q, c->num_cu);
+    if (slots <= 0) return c->fail("no decimation kernel for this order");
+    if (q == c->work_q && slots == c->work_slots && c->work.p) return 0;
+    const size_t np2 = (size_t)(16 * q) * (16 * q);
+    const size_t want = (size_t)slots * kMats * np2;
+    c->work.release();
+    if (c->work.ensure(want)) return c->fail("out of device memory (decimation workspace)");
+    WT_HIP(c, hipMemset(c->work.p, 0, want * sizeof(double2)));
+    if (c->queue.ensure(1)) return c->fail("out of device memory (queue)");
+    c->work_q = q;
+    c->work_slots = slots;
+    return 0;
+}
+
+extern "C" {
+
+int wtsurf_ctx_create(int device, wtsurf_ctx **out) {
+    if (!out) return -1;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return -1;
+    auto *c = new wtsurf_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return -2; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return -2; }
+    c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return -2;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return -2; }
+    *out = c;
+    return 0;
+}
+
+void wtsurf_ctx_destroy(wtsurf_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->hr.release(); c->irvec.release(); c->slot_begin.release();
+    c->d_top.release(); c->d_bot.release();
+    c->kpts.release(); c->omega.release(); c->phase.release(); c->hk.release();
+    c->dos_l.release(); c->dos_r.release(); c->dos_b.release();
+    c->iters.release(); c->status.release(); c->work.release(); c->queue.release();
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *wtsurf_last_error(const wtsurf_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int wtsurf_max_ndim(void) { return 16 * decimate_max_q(); }
+
+int wtsurf_set_model(wtsurf_ctx *c, int nw, int np, int nr, int ijmax, const int *irvec,
+                     const double *hr) {
+    if (!c) return -1;
+    if (nw <= 0 || np <= 0 || nr < 0 || ijmax < 0) return c->fail("invalid model sizes");
+    if (nr > 0 && (!irvec || !hr)) return c->fail("null model arrays");
+    if (int r = use_device(c)) return r;
+    // Slots d = R3 that H00/H01 can use: [-(np-1), 2np-1], clipped by ijmax.
+    const int d_lo = std::max(-(np - 1), -ijmax), d_hi = std::min(2 * np - 1, ijmax);
+    const int nslot = d_hi >= d_lo ? d_hi - d_lo + 1 : 0;
+    std::vector<int> order;
+    order.reserve(nr);
+    for (int s = 0; s < nslot; ++s)
+        for (int ir = 0; ir < nr; ++ir)
+            if (irvec[3 * ir + 2] == d_lo + s) order.push_back(ir);
+    std::vector<int> begin(nslot + 1, 0);
+    {
+        int pos = 0;
+        for (int s = 0; s < nslot; ++s) {
+            begin[s] = pos;
+            for (int ir = 0; ir < nr; ++ir)
+                if (irvec[3 * ir + 2] == d_lo + s) ++pos;
+        }
+        begin[nslot] = pos;
+    }
+    const int nsel = (int)order.size();
+    const size_t nw2 = (size_t)nw * nw;
+    std::vector<double2> hsel(std::max<size_t>((size_t)nsel * nw2, 1));
+    std::vector<int> rsel(std::max(3 * nsel, 3));
+    for (int q = 0; q < nsel; ++q) {
+        const int ir = order[q];
+        std::memcpy(&hsel[(size_t)q * nw2], hr + 2 * (size_t)ir * nw2, nw2 * sizeof(double2));
+        for (int x = 0; x < 3; ++x) rsel[3 * q + x] = irvec[3 * ir + x];
+    }
+    if (c->hr.ensure(hsel.size()) || c->irvec.ensure(rsel.size()) ||
+        c->slot_begin.ensure(begin.size()))
+        return c->fail("out of device memory (model)");
+    WT_HIP(c, hipMemcpy(c->hr.p, hsel.data(), hsel.size() * sizeof(double2), hipMemcpyHostToDevice));
+    WT_HIP(c, hipMemcpy(c->irvec.p, rsel.data(), rsel.size() * sizeof(int), hipMemcpyHostToDevice));
+    WT_HIP(c, hipMemcpy(c->slot_begin.p, begin.data(), begin.size() * sizeof(int),
+                        hipMemcpyHostToDevice));
+    c->nw = nw; c->np = np; c->nr = nsel; c->ijmax = ijmax; c->n = nw * np;
+    c->nslot = nslot; c->d_lo = d_lo;
+    c->has_model = true;
+    c->nk = 0;  // grid must be re-uploaded for the new model
+    if (!c->orbitals_set) return sync_orbitals(c);
+    return 0;
+}
+
+int wtsurf_set_stop_rule(wtsurf_ctx *c, int itermax, double accuracy) {
+    if (!c) return -1;
+    if (itermax < 1) return c->fail("itermax must be >= 1");
+    if (!(accuracy >= 0.0)) return c->fail("accuracy must be >= 0");
+    c->itermax = itermax;
+    c->accuracy = accuracy;
+    return 0;
+}
+
+int wtsurf_set_orbitals(wtsurf_ctx *c, int ntop, const int *top, int nbot, const int *bot) {
+    if (!c) return -1;
+    if (ntop < 0 || nbot < 0 || (ntop && !top) || (nbot && !bot)) return c->fail("invalid orbital lists");
+    if (c->has_model) {
+        for (int i = 0; i < ntop; ++i)
+            if (top[i] < 1 || top[i] > c->n) return c->fail("top orbital index outside 1..Ndim");
+        for (int i = 0; i < nbot; ++i)
+            if (bot[i] < 1 || bot[i] > c->nw) return c->fail("bottom orbital index outside 1..Num_wann");
+    }
+    c->top.assign(top, top + ntop);
+    c->bot.assign(bot, bot + nbot);
+    c->orbitals_set = true;
+    if (!c->has_model) return 0;
+    if (int r = use_device(c)) return r;
+    return sync_orbitals(c);
+}
+
+static int run_fourier(wtsurf_ctx *c) {
+    const int NP = 16 * padded_q(c->n);
+    if (c->phase.ensure(std::max<size_t>((size_t)c->nk * std::max(c->nr, 1), 1)) ||
+        c->hk.ensure((size_t)c->nk * 2 * NP * NP))
+        return c->fail("out of device memory (Fourier)");
+    FourierArgs fa{};
+    fa.hr = c->hr.p; fa.irvec = c->irvec.p; fa.slot_begin = c->slot_begin.p;
+    fa.nslot = c->nslot; fa.d_lo = c->d_lo; fa.nr = c->nr; fa.nw = c->nw; fa.np = c->np;
+    fa.ijmax = c->ijmax; fa.kpts = c->kpts.p; fa.nk = c->nk; fa.phase = c->phase.p;
+    fa.hk = c->hk.p; fa.NP = NP;
+    if (fourier_launch(fa, c->stream)) return c->fail("Fourier kernel launch failed");
+    return 0;
+}
+
+int wtsurf_ham_qlayer2qlayer(wtsurf_ctx *c, int nk, const double *kpts, double *h00, double *h01) {
+    if (!c) return -1;
+    if (!c->has_model) return c->fail("no model (wtsurf_set_model)");
+    if (nk < 0 || (nk > 0 && (!kpts || !h00 || !h01))) return c->fail("invalid arguments");
+    if (nk == 0) return 0;
+    if (int r = use_device(c)) return r;
+    if (c->kpts.ensure(2 * (size_t)nk)) return c->fail("out of device memory");
+    WT_HIP(c, hipMemcpy(c->kpts.p, kpts, 2 * (size_t)nk * sizeof(double), hipMemcpyHostToDevice));
+    c->nk = nk;
+    c->nomega = 0;
+    if (int r = run_fourier(c)) return r;
+    const int n = c->n, NP = 16 * padded_q(n);
+    WT_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<double2> buf((size_t)2 * NP * NP);
+    for (int ik = 0; ik < nk; ++ik) {
+        WT_HIP(c, hipMemcpy(buf.data(), c->hk.p + (size_t)ik * 2 * NP * NP,
+                            buf.size() * sizeof(double2), hipMemcpyDeviceToHost));
+        for (int m = 0; m < 2; ++m) {
+            double *dst = (m == 0 ? h00 : h01) + 2 * (size_t)ik * n * n;
+            for (int j = 0; j < n; ++j)
+                std::memcpy(dst + 2 * (size_t)j * n, &buf[(size_t)m * NP * NP + (size_t)j * NP],
+                            n * sizeof(double2));
+        }
+    }
+    c->nk = 0;  // hk no longer matches an uploaded grid
+    return 0;
+}
+
+static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const double *d_omega,
+                             double eta, double divisor, double2 *gfull, int n, int nw,
+                             const int *d_top, int ntop, const int *d_bot, int nbot) {
+    const int q = padded_q(n);
+    if (q > decimate_max_q()) return c->fail("Ndim exceeds the largest built decimation kernel");
+    if (int r = ensure_work(c, q)) return r;
+    if (c->dos_l.ensure(ninst) || c->dos_r.ensure(ninst) || c->dos_b.ensure(ninst) ||
+        c->iters.ensure(ninst) || c->status.ensure(ninst))
+        return c->fail("out of device memory (outputs)");
+    WT_HIP(c, hipMemsetAsync(c->queue.p, 0, sizeof(unsigned), c->stream));
+    DecimateArgs a{};
+    a.hk = c->hk.p; a.omega = d_omega; a.eta = nullptr; a.inst_k = nullptr; a.inst_w = nullptr;
+    a.eta0 = eta; a.nk = nk; a.nomega = nomega; a.ninst = ninst; a.n = n; a.nw = nw;
+    a.itermax = c->itermax; a.accuracy = c->accuracy;
+    a.top = d_top; a.ntop = ntop; a.bot = d_bot; a.nbot = nbot; a.divisor = divisor;
+    a.work = c->work.p; a.queue = c->queue.p;
+    a.dos_l = c->dos_l.p; a.dos_r = c->dos_r.p; a.dos_bulk = c->dos_b.p;
+    a.iters = c->iters.p; a.status = c->status.p; a.gfull = gfull;
+    const int slots = std::min(c->work_slots, ninst);
+    if (decimate_f64_launch(a, q, slots, c->stream)) return c->fail("decimation kernel launch failed");
+    return 0;
+}
+
+int wtsurf_surfgreen_1985(wtsurf_ctx *c, int n, double omega, double eta, const double *h00,
+                          const double *h01, double *gll, double *grr, double *gb, int *iters) {
+    if (!c) return -1;
+    if (n <= 0 || !h00 || !h01 || !gll || !grr || !gb) return c->fail("invalid arguments");
+    if (padded_q(n) > decimate_max_q()) return c->fail("Ndim exceeds the largest built decimation kernel");
+    if (int r = use_device(c)) return r;
+    const int NP = 16 * padded_q(n);
+    std::vector<double2> pad((size_t)2 * NP * NP, make_double2(0.0, 0.0));
+    for (int m = 0; m < 2; ++m) {
+        const double *src = m == 0 ? h00 : h01;
+        for (int j = 0; j < n; ++j)
+            std::memcpy(&pad[(size_t)m * NP * NP + (size_t)j * NP], src + 2 * (size_t)j * n,
+                        n * sizeof(double2));
+    }
+    if (c->hk.ensure(pad.size()) || c->omega.ensure(1)) return c->fail("out of device memory");
+    DevBuf<double2> gfull;
+    DevBuf<int> one;
+    if (gfull.ensure((size_t)3 * n * n) || one.ensure(1)) return c->fail("out of device memory");
+    int rc = 0;
+    do {
+        hipError_t e;
+        if ((e = hipMemcpy(c->hk.p, pad.data(), pad.size() * sizeof(double2), hipMemcpyHostToDevice)) ||
+            (e = hipMemcpy(c->omega.p, &omega, sizeof(double), hipMemcpyHostToDevice))) {
+            rc = c->hipfail(e, "upload");
+            break;
+        }
+        const int ione = 1;
+        if ((e = hipMemcpy(one.p, &ione, sizeof(int), hipMemcpyHostToDevice))) { rc = c->hipfail(e, "upload"); break; }
+        // traces are not needed here; point the lists at a valid 1-element array
+        if ((rc = launch_decimation(c, 1, 1, 1, c->omega.p, eta, 1.0, gfull.p, n, n, one.p, 0,
+                                    one.p, 0)))
+            break;
+        if ((e = hipStreamSynchronize(c->stream))) { rc = c->hipfail(e, "decimation"); break; }
+        std::vector<double2> g((size_t)3 * n * n);
+        if ((e = hipMemcpy(g.data(), gfull.p, g.size() * sizeof(double2), hipMemcpyDeviceToHost))) {
+            rc = c->hipfail(e, "download");
+            break;
+        }
+        std::memcpy(gll, &g[0], (size_t)n * n * sizeof(double2));
+        std::memcpy(grr, &g[(size_t)n * n], (size_t)n * n * sizeof(double2));
+        std::memcpy(gb, &g[(size_t)2 * n * n], (size_t)n * n * sizeof(double2));
+        if (iters && (e = hipMemcpy(iters, c->iters.p, sizeof(int), hipMemcpyDeviceToHost))) {
+            rc = c->hipfail(e, "download");
+            break;
+        }
+    } while (0);
+    gfull.release();
+    one.release();
+    c->nk = 0;  // hk was overwritten
+    return rc;
+}
+
+int wtsurf_grid_upload(wtsurf_ctx *c, int nk, const double *kpts, int nomega, const double *omega) {
+    if (!c) return -1;
+    if (!c->has_model) return c->fail("no model (wtsurf_set_model)");
+    if (nk <= 0 || nomega <= 0 || !kpts || !omega) return c->fail("invalid grid");
+    if ((long)nk * nomega > 0x7fffffffL) return c->fail("grid too large");
+    if (int r = use_device(c)) return r;
+    WT_HIP(c, hipStreamSynchronize(c->stream));  // a run in flight may still read the grid
+    if (c->kpts.ensure(2 * (size_t)nk) || c->omega.ensure(nomega)) return c->fail("out of device memory");
+    WT_HIP(c, hipMemcpy(c->kpts.p, kpts, 2 * (size_t)nk * sizeof(double), hipMemcpyHostToDevice));
+    WT_HIP(c, hipMemcpy(c->omega.p, omega, (size_t)nomega * sizeof(double), hipMemcpyHostToDevice));
+    c->nk = nk;
+    c->nomega = nomega;
+    c->ran = false;
+    return 0;
+}
+
+int wtsurf_grid_run_async(wtsurf_ctx *c, double eta, double divisor, int precision) {
+    if (!c) return -1;
+    if (!c->has_model || c->nk <= 0 || c->nomega <= 0) return c->fail("no grid (wtsurf_grid_upload)");
+    if (precision != WTSURF_FP64) return c->fail("precision mode not available in this build");
+    if (!(divisor != 0.0)) return c->fail("divisor must be non-zero");
+    if (int r = use_device(c)) return r;
+    WT_HIP(c, hipEventRecord(c->ev[0], c->stream));
+    if (int r = run_fourier(c)) return r;
+    WT_HIP(c, hipEventRecord(c->ev[1], c->stream));
+    if (int r = launch_decimation(c, c->nk * c->nomega, c->nk, c->nomega, c->omega.p, eta, divisor,
+                                  nullptr, c->n, c->nw, c->d_top.p, (int)c->top.size(), c->d_bot.p,
+                                  (int)c->bot.size()))
+        return r;
+    WT_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    c->ran = true;
+    return 0;
+}
+
+int wtsurf_sync(wtsurf_ctx *c) {
+    if (!c) return -1;
+    if (int r = use_device(c)) return r;
+    WT_HIP(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int wtsurf_grid_timing(wtsurf_ctx *c, float *ms_fourier, float *ms_decimate) {
+    if (!c) return -1;
+    if (!c->ran) return c->fail("no completed run");
+    float a = 0.f, b = 0.f;
+    WT_HIP(c, hipEventSynchronize(c->ev[2]));
+    WT_HIP(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    WT_HIP(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    if (ms_fourier) *ms_fourier = a;
+    if (ms_decimate) *ms_decimate = b;
+    return 0;
+}
+
+int wtsurf_grid_download(wtsurf_ctx *c, double *dos_l, double *dos_r, double *dos_bulk, int *iters,
+                         int *status) {
+    if (!c) return -1;
+    if (!c->ran) return c->fail("no completed run");
+    if (int r = use_device(c)) return r;
+    WT_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t ni = (size_t)c->nk * c->nomega;
+    if (dos_l) WT_HIP(c, hipMemcpy(dos_l, c->dos_l.p, ni * sizeof(double), hipMemcpyDeviceToHost));
+    if (dos_r) WT_HIP(c, hipMemcpy(dos_r, c->dos_r.p, ni * sizeof(double), hipMemcpyDeviceToHost));
+    if (dos_bulk) WT_HIP(c, hipMemcpy(dos_bulk, c->dos_b.p, ni * sizeof(double), hipMemcpyDeviceToHost));
+    if (iters) WT_HIP(c, hipMemcpy(iters, c->iters.p, ni * sizeof(int), hipMemcpyDeviceToHost));
+    if (status) WT_HIP(c, hipMemcpy(status, c->status.p, ni * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int wtsurf_spectral_grid(wtsurf_ctx *c, int nk, const double *kpts, int nomega, const double *omega,
+                         double eta, double divisor, int precision, double *dos_l, double *dos_r,
+                         double *dos_bulk, int *iters, int *status) {
+    if (int r = wtsurf_grid_upload(c, nk, kpts, nomega, omega)) return r;
+    if (int r = wtsurf_grid_run_async(c, eta, divisor, precision)) return r;
+    if (int r = wtsurf_sync(c)) return r;
+    return wtsurf_grid_download(c, dos_l, dos_r, dos_bulk, iters, status);
+}
+
+}  // extern "C"

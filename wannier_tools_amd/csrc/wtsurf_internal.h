@@ -1,0 +1,60 @@
+// Host-side declarations shared between the C-ABI layer and the kernel
+// launchers.  Not part of the public interface (include/wtsurf.h is).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace wtsurf {
+
+// Arguments of one batched decimation launch (surfgreen_1985 over many
+// (k, omega) instances, src/surfgreen.f90:7-137, with the trace epilogue of
+// src/surfstat.f90:150-160 / src/fermiarc.f90:238-248).
+struct DecimateArgs {
+    const double2 *hk;      // [nk][2][NP*NP] padded H00(k), H01(k)
+    const double *omega;    // [nomega]
+    const double *eta;      // [ninst] per-instance broadening, or null -> eta0
+    const int *inst_k;      // [ninst] k index per instance, or null -> grid order
+    const double *inst_w;   // [ninst] omega per instance (with inst_k), or null
+    double eta0;
+    int nk, nomega, ninst;  // grid mode: ninst = nk*nomega, ik = t / nomega
+    int n, nw;              // Ndim and Num_wann
+    int itermax;
+    double accuracy;
+    const int *top;         // 1-based orbital lists (TopOrbitals / BottomOrbitals)
+    int ntop;
+    const int *bot;
+    int nbot;
+    double divisor;         // 1 (surfstat) or pi (SurfaceDOSkk)
+    double2 *work;          // [slots][kMats][NP*NP], zeroed once at allocation
+    unsigned *queue;        // instance counter, zero at launch
+    double *dos_l, *dos_r, *dos_bulk;  // [ninst] in (ik + nk*iw) order
+    int *iters;             // [ninst]
+    int *status;            // [ninst] bit0 singular solve, bit1 not converged
+    double2 *gfull;         // [ninst][3][n*n] GLL, GRR, GB or null
+};
+
+constexpr int kMats = 10;   // per-instance work matrices in DecimateArgs::work
+
+int padded_q(int n);        // NP/16 for order n
+int decimate_max_q();       // largest Q the FP64 kernel is built for
+int decimate_f64_launch(const DecimateArgs &a, int q, int slots, hipStream_t s);
+int decimate_f64_slots(int q, int num_cu);
+
+// Fourier sum H00(k), H01(k) (src/ham_qlayer2qlayer.f90:45-97).
+struct FourierArgs {
+    const double2 *hr;      // [nr][nw*nw] col-major per R, Hartree, /ndegen
+    const int *irvec;       // [nr][3]
+    const int *slot_begin;  // [nslot+1] R ranges sorted by R3 (device)
+    int nslot;              // slots for R3 = d_lo .. d_lo+nslot-1
+    int d_lo;
+    int nr, nw, np, ijmax;
+    const double *kpts;     // [nk][2]
+    int nk;
+    double2 *phase;         // [nk][nr] scratch
+    double2 *hk;            // [nk][2][NP*NP] out (pad zero)
+    int NP;
+};
+int fourier_launch(const FourierArgs &a, hipStream_t s);
+
+}  // namespace wtsurf

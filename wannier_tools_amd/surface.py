@@ -1,0 +1,240 @@
+"""Host-side mirror of the reference's surface Green's function interface.
+
+Thin ctypes front over ``libwtsurf.so`` (the HIP library, C ABI in
+``include/wtsurf.h``).  The names and argument meanings follow the
+reference's Fortran routines:
+
+* ``SurfaceGF.ham_qlayer2qlayer``  -- src/ham_qlayer2qlayer.f90:1-102
+* ``SurfaceGF.surfgreen_1985``     -- src/surfgreen.f90:7-137
+* ``SurfaceGF.spectral_grid``      -- the (k, omega) loops and traces of
+  surfstat (src/surfstat.f90:123-160) and SurfaceDOSkk
+  (src/fermiarc.f90:206-248)
+
+There is no CPU fallback: constructing a ``SurfaceGF`` without the built
+library, or without a HIP device, raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .model import DECIMATION_ACCURACY, DECIMATION_ITERMAX, TBModel
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libwtsurf.so")
+
+FP64 = 0
+FP32_REFINE = 1
+_PRECISION = {"fp64": FP64, "fp32_refine": FP32_REFINE, FP64: FP64, FP32_REFINE: FP32_REFINE}
+
+STATUS_SINGULAR = 1
+STATUS_NOT_CONVERGED = 2
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+# Every symbol include/wtsurf.h declares: (name, restype, argtypes)
+SIGNATURES = {
+    "wtsurf_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "wtsurf_ctx_destroy": (None, [ctypes.c_void_p]),
+    "wtsurf_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "wtsurf_set_model": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4 + [_ip, _dp]),
+    "wtsurf_set_stop_rule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]),
+    "wtsurf_set_orbitals": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _ip, ctypes.c_int, _ip]),
+    "wtsurf_ham_qlayer2qlayer": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]),
+    "wtsurf_surfgreen_1985": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, _dp, _dp, _dp, _dp, _dp, _ip]),
+    "wtsurf_spectral_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _dp,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            _dp, _dp, _dp, _ip, _ip]),
+    "wtsurf_grid_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _dp]),
+    "wtsurf_grid_run_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int]),
+    "wtsurf_sync": (ctypes.c_int, [ctypes.c_void_p]),
+    "wtsurf_grid_timing": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp]),
+    "wtsurf_grid_download": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _ip, _ip]),
+    "wtsurf_max_ndim": (ctypes.c_int, []),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load libwtsurf.so and declare every C-ABI signature (no device calls)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(_ip)
+
+
+class WtsurfError(RuntimeError):
+    pass
+
+
+class SurfaceGF:
+    """One wtsurf context (device buffers + stream) on HIP device ``device``."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        rc = self.lib.wtsurf_ctx_create(device, ctypes.byref(h))
+        if rc != 0 or not h.value:
+            raise WtsurfError(f"wtsurf_ctx_create(device={device}) failed ({rc}): no usable HIP device")
+        self.ctx = h
+        self.n = 0
+        self.nw = 0
+        self.np = 0
+        self._grid = (0, 0)
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None) is not None and self.ctx.value:
+            self.lib.wtsurf_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = self.lib.wtsurf_last_error(self.ctx)
+            raise WtsurfError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ---- model state (module para in the reference) -----------------------
+    def set_model_arrays(self, nw: int, np_: int, irvec: np.ndarray, hr: np.ndarray,
+                         ijmax: int = 10) -> None:
+        """irvec (nr, 3) ints and hr (nr, nw, nw) complex, already in the units
+        and ndegen normalisation of HmnR_surfacecell."""
+        irvec = np.ascontiguousarray(irvec, dtype=np.int32).reshape(-1, 3)
+        hr = np.asarray(hr, dtype=np.complex128).reshape(-1, nw, nw)
+        if hr.shape[0] != irvec.shape[0]:
+            raise ValueError("irvec and hr disagree on the number of R vectors")
+        hr_cm = np.ascontiguousarray(hr.transpose(0, 2, 1))  # column-major blocks
+        self._check(self.lib.wtsurf_set_model(self.ctx, nw, np_, irvec.shape[0], ijmax,
+                                              _iptr(irvec), _dptr(hr_cm.view(np.float64))),
+                    "wtsurf_set_model")
+        self.nw, self.np, self.n = nw, np_, nw * np_
+
+    def set_model(self, model: TBModel, np_: int, ijmax: int = 10) -> None:
+        irvec, hr = model.surface_cell()
+        self.set_model_arrays(model.num_wann, np_, irvec, hr, ijmax)
+
+    def set_stop_rule(self, itermax: int = DECIMATION_ITERMAX,
+                      accuracy: float = DECIMATION_ACCURACY) -> None:
+        self._check(self.lib.wtsurf_set_stop_rule(self.ctx, int(itermax), float(accuracy)),
+                    "wtsurf_set_stop_rule")
+
+    def set_orbitals(self, top: Sequence[int], bot: Sequence[int]) -> None:
+        """1-based TopOrbitals (1..Ndim) and BottomOrbitals (1..Num_wann)."""
+        t = np.ascontiguousarray(top, dtype=np.int32)
+        b = np.ascontiguousarray(bot, dtype=np.int32)
+        self._check(self.lib.wtsurf_set_orbitals(self.ctx, t.size, _iptr(t), b.size, _iptr(b)),
+                    "wtsurf_set_orbitals")
+
+    # ---- per-call entry points ---------------------------------------------
+    def ham_qlayer2qlayer(self, kpts) -> tuple[np.ndarray, np.ndarray]:
+        """H00(k), H01(k) for kpts of shape (nk, 2) or (2,); returns arrays of
+        shape (nk, N, N) (or (N, N) for a single k)."""
+        k = np.ascontiguousarray(kpts, dtype=np.float64)
+        single = k.ndim == 1
+        k = k.reshape(-1, 2)
+        nk, n = k.shape[0], self.n
+        h00 = np.zeros((nk, n, n), dtype=np.complex128)
+        h01 = np.zeros((nk, n, n), dtype=np.complex128)
+        self._check(self.lib.wtsurf_ham_qlayer2qlayer(self.ctx, nk, _dptr(k), _dptr(h00.view(np.float64)),
+                                                      _dptr(h01.view(np.float64))),
+                    "wtsurf_ham_qlayer2qlayer")
+        h00 = h00.transpose(0, 2, 1)
+        h01 = h01.transpose(0, 2, 1)
+        return (h00[0], h01[0]) if single else (h00, h01)
+
+    def surfgreen_1985(self, omega: float, h00: np.ndarray, h01: np.ndarray,
+                       eta: float) -> tuple[np.ndarray, np.ndarray, np.ndarray, int]:
+        """(GLL, GRR, GB, iterations) for one (omega, eta) instance."""
+        h00 = np.asfortranarray(h00, dtype=np.complex128)
+        h01 = np.asfortranarray(h01, dtype=np.complex128)
+        n = h00.shape[0]
+        if h00.shape != (n, n) or h01.shape != (n, n):
+            raise ValueError("H00 and H01 must be square and of the same order")
+        g = [np.zeros((n, n), dtype=np.complex128, order="F") for _ in range(3)]
+        it = np.zeros(1, dtype=np.int32)
+        self._check(self.lib.wtsurf_surfgreen_1985(
+            self.ctx, n, float(omega), float(eta), h00.ctypes.data_as(_dp), h01.ctypes.data_as(_dp),
+            g[0].ctypes.data_as(_dp), g[1].ctypes.data_as(_dp), g[2].ctypes.data_as(_dp), _iptr(it)),
+            "wtsurf_surfgreen_1985")
+        return g[0], g[1], g[2], int(it[0])
+
+    def spectral_grid(self, kpts, omegas, eta: float, divisor: float = 1.0,
+                      precision="fp64"):
+        """dos_l, dos_r, dos_bulk (shape (nk, nomega)), iterations, status."""
+        self.upload_grid(kpts, omegas)
+        self.run_async(eta, divisor, precision)
+        self.sync()
+        return self.download()
+
+    # ---- resident split (inputs stay in device memory) -------------------
+    def upload_grid(self, kpts, omegas) -> None:
+        k = np.ascontiguousarray(kpts, dtype=np.float64).reshape(-1, 2)
+        w = np.ascontiguousarray(omegas, dtype=np.float64).ravel()
+        self._check(self.lib.wtsurf_grid_upload(self.ctx, k.shape[0], _dptr(k), w.size, _dptr(w)),
+                    "wtsurf_grid_upload")
+        self._grid = (k.shape[0], w.size)
+
+    def run_async(self, eta: float, divisor: float = 1.0, precision="fp64") -> None:
+        if precision not in _PRECISION:
+            raise ValueError(f"unknown precision {precision!r}")
+        self._check(self.lib.wtsurf_grid_run_async(self.ctx, float(eta), float(divisor),
+                                                   _PRECISION[precision]),
+                    "wtsurf_grid_run_async")
+
+    def sync(self) -> None:
+        self._check(self.lib.wtsurf_sync(self.ctx), "wtsurf_sync")
+
+    def timing(self) -> tuple[float, float]:
+        a = ctypes.c_float()
+        b = ctypes.c_float()
+        self._check(self.lib.wtsurf_grid_timing(self.ctx, ctypes.byref(a), ctypes.byref(b)),
+                    "wtsurf_grid_timing")
+        return a.value, b.value
+
+    def download(self):
+        nk, nw_ = self._grid
+        outs = [np.zeros((nk, nw_), dtype=np.float64, order="F") for _ in range(3)]
+        its = np.zeros((nk, nw_), dtype=np.int32, order="F")
+        st = np.zeros((nk, nw_), dtype=np.int32, order="F")
+        self._check(self.lib.wtsurf_grid_download(self.ctx, *(o.ctypes.data_as(_dp) for o in outs),
+                                                  its.ctypes.data_as(_ip), st.ctypes.data_as(_ip)),
+                    "wtsurf_grid_download")
+        return outs[0], outs[1], outs[2], its, st
+
+
+def max_ndim() -> int:
+    return int(load_library().wtsurf_max_ndim())
