@@ -45,7 +45,8 @@ const char *wtsurf_last_error(const wtsurf_ctx *ctx);
  * ham_qlayer2qlayer (src/readHmnR.f90:578-766, src/module.f90:957-968):
  * Num_wann = nw, Np = np, nrpts_surfacecell = nr, ijmax, irvec_surfacecell
  * as [nr][3] ints, HmnR_surfacecell as [nr] column-major nw x nw complex
- * blocks already divided by ndegen. */
+ * blocks already divided by ndegen.  Resets the orbital lists to their
+ * defaults (wtsurf_set_orbitals applies to the current model). */
 int wtsurf_set_model(wtsurf_ctx *ctx, int nw, int np, int nr, int ijmax,
                      const int *irvec, const double *hr);
 

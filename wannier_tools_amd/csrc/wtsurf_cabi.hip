@@ -69,9 +69,9 @@ struct wtsurf_ctx {
     DevBuf<double> dos_l, dos_r, dos_b;
     DevBuf<int> iters, status;
     // decimation workspace
-    DevBuf<double2> work;
+    DevBuf<unsigned char> work;
     DevBuf<unsigned> queue;
-    int work_q = 0, work_slots = 0;
+    int work_slots = 0;
     bool ran = false;
 
     int fail(const std::string &m) {
@@ -120,17 +120,16 @@ static int sync_orbitals(wtsurf_ctx *c) {
     return 0;
 }
 
-static int ensure_work(wtsurf_ctx *c, int q) {
-    const int slots = decimate_f64_slots(q, c->num_cu);
+static int ensure_work(wtsurf_ctx *c, int q, int precision) {
+    const int slots = decimate_slots(q, precision, c->num_cu);
     if (slots <= 0) return c->fail("no decimation kernel for this order");
-    if (q == c->work_q && slots == c->work_slots && c->work.p) return 0;
-    const size_t np2 = (size_t)(16 * q) * (16 * q);
-    const size_t want = (size_t)slots * kMats * np2;
-    c->work.release();
-    if (c->work.ensure(want)) return c->fail("out of device memory (decimation workspace)");
-    WT_HIP(c, hipMemset(c->work.p, 0, want * sizeof(double2)));
+    const size_t want = (size_t)slots * decimate_slot_bytes(q, precision);
+    if (want > c->work.n) {
+        c->work.release();
+        if (c->work.ensure(want)) return c->fail("out of device memory (decimation workspace)");
+        WT_HIP(c, hipMemset(c->work.p, 0, want));
+    }
     if (c->queue.ensure(1)) return c->fail("out of device memory (queue)");
-    c->work_q = q;
     c->work_slots = slots;
     return 0;
 }
@@ -221,8 +220,8 @@ int wtsurf_set_model(wtsurf_ctx *c, int nw, int np, int nr, int ijmax, const int
     c->nslot = nslot; c->d_lo = d_lo;
     c->has_model = true;
     c->nk = 0;  // grid must be re-uploaded for the new model
-    if (!c->orbitals_set) return sync_orbitals(c);
-    return 0;
+    c->orbitals_set = false;  // orbital lists belong to a model: back to the defaults
+    return sync_orbitals(c);
 }
 
 int wtsurf_set_stop_rule(wtsurf_ctx *c, int itermax, double accuracy) {
@@ -295,10 +294,11 @@ int wtsurf_ham_qlayer2qlayer(wtsurf_ctx *c, int nk, const double *kpts, double *
 
 static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const double *d_omega,
                              double eta, double divisor, double2 *gfull, int n, int nw,
-                             const int *d_top, int ntop, const int *d_bot, int nbot) {
+                             const int *d_top, int ntop, const int *d_bot, int nbot,
+                             int precision) {
     const int q = padded_q(n);
     if (q > decimate_max_q()) return c->fail("Ndim exceeds the largest built decimation kernel");
-    if (int r = ensure_work(c, q)) return r;
+    if (int r = ensure_work(c, q, precision)) return r;
     if (c->dos_l.ensure(ninst) || c->dos_r.ensure(ninst) || c->dos_b.ensure(ninst) ||
         c->iters.ensure(ninst) || c->status.ensure(ninst))
         return c->fail("out of device memory (outputs)");
@@ -312,7 +312,7 @@ static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const
     a.dos_l = c->dos_l.p; a.dos_r = c->dos_r.p; a.dos_bulk = c->dos_b.p;
     a.iters = c->iters.p; a.status = c->status.p; a.gfull = gfull;
     const int slots = std::min(c->work_slots, ninst);
-    if (decimate_f64_launch(a, q, slots, c->stream)) return c->fail("decimation kernel launch failed");
+    if (decimate_launch(a, q, precision, slots, c->stream)) return c->fail("decimation kernel launch failed");
     return 0;
 }
 
@@ -346,7 +346,7 @@ int wtsurf_surfgreen_1985(wtsurf_ctx *c, int n, double omega, double eta, const 
         if ((e = hipMemcpy(one.p, &ione, sizeof(int), hipMemcpyHostToDevice))) { rc = c->hipfail(e, "upload"); break; }
         // traces are not needed here; point the lists at a valid 1-element array
         if ((rc = launch_decimation(c, 1, 1, 1, c->omega.p, eta, 1.0, gfull.p, n, n, one.p, 0,
-                                    one.p, 0)))
+                                    one.p, 0, WTSURF_FP64)))
             break;
         if ((e = hipStreamSynchronize(c->stream))) { rc = c->hipfail(e, "decimation"); break; }
         std::vector<double2> g((size_t)3 * n * n);
@@ -387,7 +387,8 @@ int wtsurf_grid_upload(wtsurf_ctx *c, int nk, const double *kpts, int nomega, co
 int wtsurf_grid_run_async(wtsurf_ctx *c, double eta, double divisor, int precision) {
     if (!c) return -1;
     if (!c->has_model || c->nk <= 0 || c->nomega <= 0) return c->fail("no grid (wtsurf_grid_upload)");
-    if (precision != WTSURF_FP64) return c->fail("precision mode not available in this build");
+    if (precision != WTSURF_FP64 && precision != WTSURF_FP32_REFINE)
+        return c->fail("unknown precision mode");
     if (!(divisor != 0.0)) return c->fail("divisor must be non-zero");
     if (int r = use_device(c)) return r;
     WT_HIP(c, hipEventRecord(c->ev[0], c->stream));
@@ -395,7 +396,7 @@ int wtsurf_grid_run_async(wtsurf_ctx *c, double eta, double divisor, int precisi
     WT_HIP(c, hipEventRecord(c->ev[1], c->stream));
     if (int r = launch_decimation(c, c->nk * c->nomega, c->nk, c->nomega, c->omega.p, eta, divisor,
                                   nullptr, c->n, c->nw, c->d_top.p, (int)c->top.size(), c->d_bot.p,
-                                  (int)c->bot.size()))
+                                  (int)c->bot.size(), precision))
         return r;
     WT_HIP(c, hipEventRecord(c->ev[2], c->stream));
     c->ran = true;

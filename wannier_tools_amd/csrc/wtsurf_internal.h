@@ -26,7 +26,7 @@ struct DecimateArgs {
     const int *bot;
     int nbot;
     double divisor;         // 1 (surfstat) or pi (SurfaceDOSkk)
-    double2 *work;          // [slots][kMats][NP*NP], zeroed once at allocation
+    void *work;             // [slots][decimate_slot_bytes], zeroed once at allocation
     unsigned *queue;        // instance counter, zero at launch
     double *dos_l, *dos_r, *dos_bulk;  // [ninst] in (ik + nk*iw) order
     int *iters;             // [ninst]
@@ -34,12 +34,12 @@ struct DecimateArgs {
     double2 *gfull;         // [ninst][3][n*n] GLL, GRR, GB or null
 };
 
-constexpr int kMats = 10;   // per-instance work matrices in DecimateArgs::work
-
 int padded_q(int n);        // NP/16 for order n
-int decimate_max_q();       // largest Q the FP64 kernel is built for
-int decimate_f64_launch(const DecimateArgs &a, int q, int slots, hipStream_t s);
-int decimate_f64_slots(int q, int num_cu);
+int decimate_max_q();       // largest Q the decimation kernels are built for
+// precision: 0 = FP64, 1 = FP32 inverse/GEMMs with FP64 accumulation and finals
+int decimate_slots(int q, int precision, int num_cu);
+size_t decimate_slot_bytes(int q, int precision);
+int decimate_launch(const DecimateArgs &a, int q, int precision, int slots, hipStream_t s);
 
 // Fourier sum H00(k), H01(k) (src/ham_qlayer2qlayer.f90:45-97).
 struct FourierArgs {
