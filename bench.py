@@ -135,7 +135,20 @@ def main() -> int:
     ap.add_argument("--cpu-workers", type=int, default=16)
     ap.add_argument("--cpu-per-worker", type=int, default=400)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the surface Green's function path is not "
@@ -176,8 +189,11 @@ def main() -> int:
                       ((dl, dl64), (dr, dr64), (db, db64)))
             med = float(np.median(np.concatenate([np.abs(x - y).ravel() / np.abs(y).ravel() for x, y in
                                                   ((dl, dl64), (dr, dr64), (db, db64))])))
+            rel = np.concatenate([np.abs(x - y).ravel() / np.abs(y).ravel() for x, y in
+                                  ((dl, dl64), (dr, dr64), (db, db64))])
             res["fp32_refine"] = {"s_per_step": s32, "solves_per_s": ninst / s32, "decimate_ms": dec32,
                                   "max_rel_err_A": err, "median_rel_err_A": med,
+                                  "frac_within_1e-6": float(np.mean(rel <= 1e-6)),
                                   "speedup_vs_fp64": s64 / s32}
             log(f"[bench] fp32_refine: {ninst / s32:.1f} solves/s, max rel err {err:.2e}")
             if err <= 1e-6 and s32 < best[1]:
@@ -185,6 +201,13 @@ def main() -> int:
 
     name, s_step, dec_ms, peak = best
     achieved = flops / (dec_ms * 1e-3) / 1e12
+    # HBM bytes per launch of this kernel on this workload, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a 1 GiB copy calibration)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "decimate_fp64_ndim72_512x512.traffic.json")
+    if name == "fp64" and (args.nk, args.nomega) == (512, 512) and os.path.exists(tpath):
+        t = json.load(open(tpath))
+        traffic = t["fetch_bytes"] + t["write_bytes"]
     out = {
         "metric": "G(k,w) solves/s @Ndim=72: FP32+refine vs FP64 GPU, err<=1e-6; %MFMA peak",
         "value": ninst / s_step,
@@ -204,8 +227,8 @@ def main() -> int:
                    "eta_ev": 1e-3, "stop_rule": "sum|alpha|<=1e-16 Ha, itermax 100",
                    "path": name},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "decimate_f64_kernel<5>" if name == "fp64" else "decimate_f32",
+                     "frac": achieved / peak, "traffic": traffic,
+                     "kernel": "decimate_kernel<5, double>" if name == "fp64" else "decimate_kernel<5, float>",
                      "flops_per_launch": flops},
         "paths": res,
     }

@@ -116,18 +116,15 @@ __device__ bool gj_inverse(unsigned char *lds, Common<Q> &cm, const double2 *__r
         if (wave == 0) {
             const int r0 = lane < NP ? lane : 0, r1 = lane + 64;
             const bool has0 = lane < n;
-            const bool has1 = NP > 64 && r1 < n;
+            const bool two = NP > 64 && uniform(n > 64 ? 1 : 0);  // second rows exist
+            const bool has1 = two && r1 < n;
             C a0[kPanel], a1[kPanel];
 #pragma unroll
             for (int j = 0; j < kPanel; ++j) {
                 a0[j] = S[c0 + j][r0];
                 if (!has0) a0[j].x = a0[j].y = 0;
-                if (has1) {
-                    a1[j] = S[c0 + j][r1];
-                } else {
-                    a1[j].x = 0;
-                    a1[j].y = 0;
-                }
+                a1[j].x = a1[j].y = 0;
+                if (has1) a1[j] = S[c0 + j][r1];
             }
             bool free0 = has0 && cm.pivstep[r0] < 0;
             bool free1 = has1 && cm.pivstep[has1 ? r1 : 0] < 0;
@@ -139,16 +136,16 @@ __device__ bool gj_inverse(unsigned char *lds, Common<Q> &cm, const double2 *__r
                 if (c >= n) break;
                 const double v0 = free0 ? (double)(fabs(a0[j].x) + fabs(a0[j].y)) : -1.0;
                 const double v1 = free1 ? (double)(fabs(a1[j].x) + fabs(a1[j].y)) : -1.0;
-                const double best = wave_max_f64(fmax(v0, v1));
+                const double best = wave_max_f64(two ? fmax(v0, v1) : v0);
                 if (!(best > 0.0)) {  // zero (or NaN) pivot column: singular
                     sing = true;
                     break;
                 }
                 // lowest row attaining the maximum: rows 0..63 before 64..127
                 const unsigned long long m0 = __ballot(v0 == best);
-                const unsigned long long m1 = __ballot(v1 == best);
-                const int slot = m0 ? 0 : 1;
-                const int pl = __ffsll((long long)(m0 ? m0 : m1)) - 1;
+                const unsigned long long m1 = two ? __ballot(v1 == best) : 0ull;
+                const int slot = uniform(m0 ? 0 : 1);
+                const int pl = uniform(__ffsll((long long)(m0 ? m0 : m1)) - 1);
                 const int p = pl + 64 * slot;
                 // this pivot row's entries outside the panel, for the MFMA update
 #pragma unroll
@@ -162,36 +159,54 @@ __device__ bool gj_inverse(unsigned char *lds, Common<Q> &cm, const double2 *__r
                         T[j][col] = tv;
                     }
                 }
-                // pivot row, normalised: entry j -> 1/pivot, others scaled by it
+                // pivot row from lane pl, normalised: entry j -> 1/pivot, others scaled
                 C rowv[kPanel];
+                if (slot == 0) {
 #pragma unroll
-                for (int jj = 0; jj < kPanel; ++jj) {
-                    const C src = slot ? a1[jj] : a0[jj];
-                    rowv[jj].x = P::readlane(src.x, pl);
-                    rowv[jj].y = P::readlane(src.y, pl);
+                    for (int jj = 0; jj < kPanel; ++jj) {
+                        rowv[jj].x = P::readlane(a0[jj].x, pl);
+                        rowv[jj].y = P::readlane(a0[jj].y, pl);
+                    }
+                } else {
+#pragma unroll
+                    for (int jj = 0; jj < kPanel; ++jj) {
+                        rowv[jj].x = P::readlane(a1[jj].x, pl);
+                        rowv[jj].y = P::readlane(a1[jj].y, pl);
+                    }
                 }
-                const C r = crecip(rowv[j]);
+                const C r = crecip_uniform(rowv[j]);
 #pragma unroll
                 for (int jj = 0; jj < kPanel; ++jj) rowv[jj] = (jj == j) ? r : cmul(rowv[jj], r);
-                // eliminate column c from every other row; the pivot row takes rowv
-                const bool own0 = slot == 0 && lane == pl, own1 = slot == 1 && lane == pl;
-                const C f0 = a0[j], f1 = a1[j];
+                // eliminate column c from every row (pivot row overwritten below)
+                {
+                    const C f = a0[j];
+                    a0[j].x = a0[j].y = 0;
 #pragma unroll
-                for (int jj = 0; jj < kPanel; ++jj) {
-                    C x0 = a0[jj], x1 = a1[jj];
-                    if (jj == j) {
-                        x0.x = x0.y = 0;
-                        x1.x = x1.y = 0;
+                    for (int jj = 0; jj < kPanel; ++jj) {
+                        a0[jj].x -= f.x * rowv[jj].x - f.y * rowv[jj].y;
+                        a0[jj].y -= f.x * rowv[jj].y + f.y * rowv[jj].x;
                     }
-                    x0.x -= f0.x * rowv[jj].x - f0.y * rowv[jj].y;
-                    x0.y -= f0.x * rowv[jj].y + f0.y * rowv[jj].x;
-                    x1.x -= f1.x * rowv[jj].x - f1.y * rowv[jj].y;
-                    x1.y -= f1.x * rowv[jj].y + f1.y * rowv[jj].x;
-                    a0[jj] = own0 ? rowv[jj] : x0;
-                    a1[jj] = own1 ? rowv[jj] : x1;
                 }
-                if (own0) free0 = false;
-                if (own1) free1 = false;
+                if (two) {
+                    const C f = a1[j];
+                    a1[j].x = a1[j].y = 0;
+#pragma unroll
+                    for (int jj = 0; jj < kPanel; ++jj) {
+                        a1[jj].x -= f.x * rowv[jj].x - f.y * rowv[jj].y;
+                        a1[jj].y -= f.x * rowv[jj].y + f.y * rowv[jj].x;
+                    }
+                }
+                if (lane == pl) {
+                    if (slot == 0) {
+#pragma unroll
+                        for (int jj = 0; jj < kPanel; ++jj) a0[jj] = rowv[jj];
+                        free0 = false;
+                    } else {
+#pragma unroll
+                        for (int jj = 0; jj < kPanel; ++jj) a1[jj] = rowv[jj];
+                        free1 = false;
+                    }
+                }
                 if (lane == 0) {
                     cm.pivrow[c] = p;
                     cm.pivstep[p] = c;

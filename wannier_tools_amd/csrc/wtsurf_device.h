@@ -100,6 +100,25 @@ __device__ __forceinline__ C crecip(C z) {
     return out;
 }
 
+// crecip for a value every lane holds identically: the scaling branch is
+// taken as one uniform branch instead of evaluating both sides.
+template <typename C>
+__device__ __forceinline__ C crecip_uniform(C z) {
+    C out;
+    if (uniform(fabs(z.x) >= fabs(z.y) ? 1 : 0)) {
+        const auto r = z.y / z.x;
+        const auto inv = decltype(r)(1) / (z.x + z.y * r);
+        out.x = inv;
+        out.y = -r * inv;
+    } else {
+        const auto r = z.x / z.y;
+        const auto inv = decltype(r)(1) / (z.y + z.x * r);
+        out.x = r * inv;
+        out.y = -inv;
+    }
+    return out;
+}
+
 // Per-precision MFMA: one 16x16x4 step, D += A B, with the operand layout
 //   A[i][k]: lane i + 16k;  B[k][j]: lane j + 16k;  D[i][j]: lane j + 16*q,
 // element v, where (checked on the device with known-answer products)
