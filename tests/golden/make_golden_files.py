@@ -1,0 +1,146 @@
+"""Golden output files of the reference program for the file-level tests.
+
+Runs only where the reference tree exists: builds oracle/_ref/wt.x (the whole
+reference program, compiled unchanged from its sources by oracle/Makefile),
+runs it on small synthetic inputs and stores, per case, the inputs (wt.in,
+*_hr.dat) and the reference's dos.dat_{l,r,bulk} and arc.dat_{l,r,bulk}
+under tests/golden/files_<case>/.
+
+    python tests/golden/make_golden_files.py
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from wannier_tools_amd import model as M  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+WTX = os.path.join(ROOT, "oracle", "_ref", "wt.x")
+OUTPUTS = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk"]
+
+WT_IN = """&TB_FILE
+Hrfile = '{hr}'
+/
+&CONTROL
+SlabSS_calc = T
+SlabArc_calc = T
+/
+&SYSTEM
+SOC = {soc}
+E_FERMI = {efermi}
+/
+&PARAMETERS
+Eta_Arc = {eta}
+E_arc = {earc}
+OmegaNum = {nomega}
+OmegaMin = -1.0
+OmegaMax = 1.0
+Nk1 = {nk1}
+Nk2 = {nk2}
+NP = {np}
+/
+LATTICE
+Angstrom
+{a1}
+{a2}
+{a3}
+ATOM_POSITIONS
+{natoms}
+Direct
+{atoms}
+PROJECTORS
+{nprojs}
+{projlines}
+SURFACE
+1 0 0
+0 1 0
+KPATH_SLAB
+2
+G 0.0 0.0 X 0.5 0.0
+X 0.5 0.0 M 0.5 0.5
+KPLANE_SLAB
+-0.5 -0.5
+1.0 0.0
+0.0 1.0
+{surface_atoms}"""
+
+PROJ = ["s", "pz", "px", "py", "dz2", "dxz", "dyz", "dxy", "dx2-y2"]
+
+
+def write_case(d: str, name: str, mdl: M.TBModel, soc: int, nprojs: list[int], np_: int,
+               eta: float, earc: float, efermi: float, nomega: int, nk1: int, nk2: int,
+               lattice, surface_atoms: str = "") -> None:
+    hrname = f"{name}_hr.dat"
+    M.write_hr(os.path.join(d, hrname), mdl)
+    atoms = "\n".join(f"C {0.1 * i:.3f} {0.05 * i:.3f} {0.2 * i:.3f}" for i in range(len(nprojs)))
+    projlines = "\n".join("C " + " ".join(PROJ[:p]) for p in nprojs)
+    text = WT_IN.format(hr=hrname, soc=soc, efermi=efermi, eta=eta, earc=earc, nomega=nomega,
+                        nk1=nk1, nk2=nk2, np=np_, a1=lattice[0], a2=lattice[1], a3=lattice[2],
+                        natoms=len(nprojs), atoms=atoms, nprojs=" ".join(map(str, nprojs)),
+                        projlines=projlines, surface_atoms=surface_atoms)
+    with open(os.path.join(d, "wt.in"), "w") as f:
+        f.write(text)
+
+
+def run_reference(d: str) -> None:
+    """One reference process per driver: SurfaceDOSkk accumulates dos_bulk into an
+    array it never zeroes (src/fermiarc.f90:247 on the allocation of :77-82), so
+    after surfstat has run in the same process arc.dat_bulk carries stale heap
+    values.  A process of its own gives SurfaceDOSkk fresh (zero) memory."""
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1")
+    text = open(os.path.join(d, "wt.in")).read()
+    for flag, out in (("SlabSS_calc", "dos.dat_l"), ("SlabArc_calc", "arc.dat_l")):
+        one = text.replace("SlabSS_calc = T", f"SlabSS_calc = {'T' if flag == 'SlabSS_calc' else 'F'}")
+        one = one.replace("SlabArc_calc = T", f"SlabArc_calc = {'T' if flag == 'SlabArc_calc' else 'F'}")
+        with open(os.path.join(d, "wt.in"), "w") as f:
+            f.write(one)
+        r = subprocess.run([WTX], cwd=d, env=env, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0 or not os.path.exists(os.path.join(d, out)):
+            raise RuntimeError(f"reference wt.x failed in {d}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+    with open(os.path.join(d, "wt.in"), "w") as f:
+        f.write(text)
+
+
+CASES = {
+    # 8 orbitals on one atom, Np = 2 (Ndim = 16), non-cubic lattice, E_fermi shift
+    "ndim16": dict(mdl=M.model_ndim16, soc=0, nprojs=[8], np_=2, eta=0.002, earc=0.1,
+                   efermi=0.05, nomega=21, nk1=11, nk2=7,
+                   lattice=["1.1 0.0 0.0", "0.3 0.9 0.0", "0.0 0.1 1.3"]),
+    # spinful: 2 atoms x (3 + 2) orbitals x spin = 10, Np = 3 (Ndim = 30),
+    # SURFACE_ATOMS picks different atoms for the two surfaces
+    "soc_ndim30": dict(mdl=lambda: M.synthetic_model(10, 1, 0.3, M.decay_l1, seed=11,
+                                                     soc_sigma=0.05, name="soc10"),
+                       soc=1, nprojs=[3, 2], np_=3, eta=0.004, earc=-0.2, efermi=0.0,
+                       nomega=15, nk1=9, nk2=5,
+                       lattice=["1.0 0.0 0.0", "0.0 1.0 0.0", "0.0 0.0 1.0"],
+                       surface_atoms="SURFACE_ATOMS\n2\n1\n"),
+}
+
+
+def main() -> None:
+    subprocess.run(["make", "-s", "-j8", "-C", os.path.join(ROOT, "oracle"), "refexe"], check=True)
+    for name, c in CASES.items():
+        dst = os.path.join(OUT, f"files_{name}")
+        with tempfile.TemporaryDirectory() as d:
+            write_case(d, name, c["mdl"](), c["soc"], c["nprojs"], c["np_"], c["eta"], c["earc"],
+                       c["efermi"], c["nomega"], c["nk1"], c["nk2"], c["lattice"],
+                       c.get("surface_atoms", ""))
+            run_reference(d)
+            shutil.rmtree(dst, ignore_errors=True)
+            os.makedirs(dst)
+            for fn in ["wt.in", f"{name}_hr.dat"] + OUTPUTS:
+                shutil.copy(os.path.join(d, fn), os.path.join(dst, fn))
+        print(f"wrote {dst}")
+
+
+if __name__ == "__main__":
+    main()

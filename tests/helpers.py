@@ -49,3 +49,36 @@ def chain_bulk_g(z, e0, t):
     s = np.sqrt(x * x - 4 * t * t + 0j)
     g = 1.0 / s
     return g if g.imag <= 0 else -g
+
+
+# ---- reference output files (dos.dat_*, arc.dat_*) -------------------------
+def read_table(path):
+    """(comment lines, numeric rows as float arrays, blank-line positions)."""
+    comments, rows, blanks = [], [], []
+    with open(path) as f:
+        for i, line in enumerate(f):
+            s = line.strip()
+            if not s:
+                blanks.append(len(rows))
+            elif s.startswith("#"):
+                comments.append(line.rstrip("\n"))
+            else:
+                rows.append([float(x) for x in s.split()])
+    return comments, np.array(rows), blanks
+
+
+def compare_tables(mine, ref, value_atol=2e-7, value_rtol=2e-7):
+    """Same headers, same row/blank structure, coordinates equal to the last
+    printed digit, values within the tolerance (on the printed numbers)."""
+    cm, rm, bm = read_table(mine)
+    cr, rr, br = read_table(ref)
+    assert cm == cr, f"{mine}: header lines differ"
+    assert rm.shape == rr.shape, f"{mine}: {rm.shape} rows vs {rr.shape}"
+    assert bm == br, f"{mine}: blank-line positions differ"
+    coords = 2
+    assert np.allclose(rm[:, :coords], rr[:, :coords], rtol=1e-7, atol=1e-8), f"{mine}: k/E columns"
+    err = np.abs(rm[:, coords:] - rr[:, coords:])
+    bound = value_atol + value_rtol * np.abs(rr[:, coords:])
+    worst = float(np.max(err / bound)) if err.size else 0.0
+    assert worst <= 1.0, f"{mine}: values differ, worst err/bound {worst:.3g}"
+    return float(np.max(err)) if err.size else 0.0

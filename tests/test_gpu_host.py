@@ -1,0 +1,53 @@
+"""The Fortran host on the GPU against the reference program's own files.
+
+* wannier_tools_amd/bin/wt_surf -- the standalone driver (wt.in + *_hr.dat in,
+  dos.dat_* / arc.dat_* out) running surfstat and SurfaceDOSkk through
+  libwtsurf;
+* oracle/_ref/wt_gpu.x (where built) -- the reference program itself with
+  the drop-in wtsurf_dropin.f90 linked ahead of its own surfstat /
+  SurfaceDOSkk / surfgreen_1985 / ham_qlayer2qlayer.
+
+Each must reproduce the reference's printed numbers (tests/golden/files_*)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ROOT
+from helpers import compare_tables
+
+pytestmark = pytest.mark.gpu
+
+CASES = ["ndim16", "soc_ndim30"]
+FILES = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk"]
+WT_SURF = os.path.join(ROOT, "wannier_tools_amd", "bin", "wt_surf")
+WT_GPU = os.path.join(ROOT, "oracle", "_ref", "wt_gpu.x")
+
+
+def run_case(exe, name, tmp_path):
+    src = os.path.join(GOLDEN, f"files_{name}")
+    for fn in ("wt.in", f"{name}_hr.dat"):
+        shutil.copy(os.path.join(src, fn), tmp_path / fn)
+    print("run", os.path.basename(exe), name, flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, OPENBLAS_NUM_THREADS="1"))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    worst = {}
+    for fn in FILES:
+        worst[fn] = compare_tables(str(tmp_path / fn), os.path.join(src, fn))
+    print("  max abs diff of printed values:", worst, flush=True)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_wt_surf_reproduces_reference_files(name, tmp_path):
+    assert os.path.exists(WT_SURF), "wt_surf not built (run __graft_entry__.build())"
+    run_case(WT_SURF, name, tmp_path)
+    wt_out = open(tmp_path / "WT.out").read()
+    assert "Time cost for SlabSS_calc" in wt_out and "Time cost for SlabArc" in wt_out
+
+
+@pytest.mark.skipif(not os.path.exists(WT_GPU), reason="reference program with the drop-in not built")
+@pytest.mark.parametrize("name", CASES)
+def test_reference_program_with_gpu_drop_in(name, tmp_path):
+    run_case(WT_GPU, name, tmp_path)

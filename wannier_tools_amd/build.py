@@ -1,8 +1,9 @@
-"""Build the wtsurf HIP library (libwtsurf.so) for gfx950, in-tree.
+"""Build the wtsurf HIP library (libwtsurf.so) for gfx950 and the Fortran host, in-tree.
 
 hipcc compiles each translation unit of csrc/ for --offload-arch=gfx950 and
-links them into wannier_tools_amd/libwtsurf.so.  Packed-FP32 instruction
-selection is switched off for all device code.
+links them into wannier_tools_amd/libwtsurf.so (packed-FP32 instruction
+selection is switched off for all device code); amdflang then builds the
+Fortran host wannier_tools_amd/bin/wt_surf against it.
 """
 from __future__ import annotations
 
@@ -35,6 +36,36 @@ def _compile(src: str) -> str:
     return obj
 
 
+FC = os.environ.get("WTSURF_FC", "/opt/rocm/llvm/bin/amdflang")
+FORTRAN = os.path.join(HERE, "fortran")
+BIN = os.path.join(HERE, "bin")
+# order matters: modules before their users
+FSOURCES = ["para_subset.f90", "wtsurf_cabi.f90", "wtsurf_dropin.f90", "wt_surf.f90"]
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """The Fortran host: wt_surf linked against libwtsurf.so (rpath to it)."""
+    mod = os.path.join(BUILD, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    objs = []
+    for src in FSOURCES:
+        obj = os.path.join(mod, os.path.splitext(src)[0] + ".o")
+        cmd = [FC, "-O2", "-fPIC", "-module-dir", mod, "-c", os.path.join(FORTRAN, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {src}:\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    exe = os.path.join(BIN, "wt_surf")
+    cmd = [FC, "-o", exe, *objs, f"-L{HERE}", "-lwtsurf", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link of wt_surf failed:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {exe}")
+    return exe
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -47,6 +78,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_fortran(verbose)
     return LIB
 
 

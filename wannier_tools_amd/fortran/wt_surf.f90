@@ -1,0 +1,507 @@
+! wt_surf: standalone driver for the surface Green's function path on the GPU.
+!
+! Run in a WannierTools input directory: reads wt.in (the part the path
+! uses) and the Wannier90 *_hr.dat it names, then runs SlabSS_calc (surfstat,
+! dos.dat_*) and/or SlabArc_calc (SurfaceDOSkk, arc.dat_*) exactly as the
+! reference's main program dispatches them (src/main.f90:714-733), and
+! reports the time of each in WT.out.
+!
+! Input subset (names, defaults and units as src/readinput.f90):
+!   &TB_FILE    Hrfile, Particle
+!   &CONTROL    SlabSS_calc, SlabArc_calc (other flags are accepted and ignored)
+!   &SYSTEM     SOC, E_FERMI
+!   &PARAMETERS Eta_Arc, E_arc, OmegaNum, OmegaMin, OmegaMax, Nk1, Nk2, NP
+!   cards       LATTICE, ATOM_POSITIONS, PROJECTORS, SURFACE, KPATH_SLAB,
+!               KPLANE_SLAB, SURFACE_ATOMS
+! The SURFACE card must keep the crystal's own (001) cell (R1 = a1, R2 = a2,
+! the default); other surfaces need the cell remap of get_hmnr_cell
+! (src/readHmnR.f90:578-766), which this program does not carry.
+module wt_surf_input
+   use para
+   implicit none
+   private
+   public :: read_wt_in, read_hr
+
+   integer, parameter :: maxkv = 512
+   character(len=64) :: keys(maxkv)
+   character(len=256) :: vals(maxkv)
+   integer :: nkv = 0
+
+contains
+
+   function lower(s) result(t)
+      character(len=*), intent(in) :: s
+      character(len=len(s)) :: t
+      integer :: i, c
+      t = s
+      do i = 1, len(s)
+         c = iachar(s(i:i))
+         if (c >= iachar('A') .and. c <= iachar('Z')) t(i:i) = achar(c + 32)
+      end do
+   end function lower
+
+   !> Collect name = value pairs of every namelist group in the file.
+   subroutine scan_namelists(fname)
+      character(len=*), intent(in) :: fname
+      character(len=1024) :: line
+      character(len=8192) :: buf
+      integer :: u, ios, p, q, e
+      logical :: ingroup, inquote
+      nkv = 0
+      open(newunit=u, file=fname, status='old', action='read', iostat=ios)
+      if (ios /= 0) then
+         write(*, '(3a)') ' >>> ERROR: cannot open ', trim(fname), ' in the current directory'
+         stop 1
+      end if
+      ingroup = .false.
+      buf = ''
+      do
+         read(u, '(a)', iostat=ios) line
+         if (ios /= 0) exit
+         p = index(line, '!')
+         if (p > 0) line = line(1:p - 1)
+         line = adjustl(line)
+         if (.not. ingroup) then
+            if (line(1:1) == '&') then
+               ingroup = .true.
+               buf = ''
+               p = index(line, ' ')
+               line = line(p:)
+            else
+               cycle
+            end if
+         end if
+         ! a '/' outside quotes ends the group
+         inquote = .false.
+         e = 0
+         do q = 1, len_trim(line)
+            if (line(q:q) == "'" .or. line(q:q) == '"') inquote = .not. inquote
+            if (line(q:q) == '/' .and. .not. inquote) then
+               e = q
+               exit
+            end if
+         end do
+         if (e > 0) then
+            buf = trim(buf)//', '//line(1:e - 1)
+            call split_pairs(buf)
+            ingroup = .false.
+         else
+            buf = trim(buf)//', '//trim(line)
+         end if
+      end do
+      close(u)
+   end subroutine scan_namelists
+
+   subroutine split_pairs(buf)
+      character(len=*), intent(in) :: buf
+      integer :: i, start, eq
+      logical :: inquote
+      character(len=512) :: tok
+      start = 1
+      inquote = .false.
+      do i = 1, len_trim(buf) + 1
+         if (i <= len_trim(buf)) then
+            if (buf(i:i) == "'" .or. buf(i:i) == '"') inquote = .not. inquote
+            if (inquote .or. buf(i:i) /= ',') cycle
+         end if
+         tok = adjustl(buf(start:i - 1))
+         start = i + 1
+         eq = index(tok, '=')
+         if (eq == 0 .or. nkv >= maxkv) cycle
+         nkv = nkv + 1
+         keys(nkv) = lower(adjustl(tok(1:eq - 1)))
+         vals(nkv) = adjustl(tok(eq + 1:))
+      end do
+   end subroutine split_pairs
+
+   logical function find(key, val)
+      character(len=*), intent(in) :: key
+      character(len=*), intent(out) :: val
+      integer :: i
+      find = .false.
+      val = ''
+      do i = nkv, 1, -1
+         if (trim(keys(i)) == lower(key)) then
+            val = vals(i)
+            find = .true.
+            return
+         end if
+      end do
+   end function find
+
+   subroutine get_real(key, x)
+      character(len=*), intent(in) :: key
+      real(dp), intent(inout) :: x
+      character(len=256) :: v
+      integer :: i, ios
+      if (.not. find(key, v)) return
+      do i = 1, len_trim(v)
+         if (v(i:i) == 'd' .or. v(i:i) == 'D') v(i:i) = 'e'
+      end do
+      read(v, *, iostat=ios) x
+      if (ios /= 0) call bad(key, v)
+   end subroutine get_real
+
+   subroutine get_int(key, x)
+      character(len=*), intent(in) :: key
+      integer, intent(inout) :: x
+      character(len=256) :: v
+      integer :: ios
+      if (.not. find(key, v)) return
+      read(v, *, iostat=ios) x
+      if (ios /= 0) call bad(key, v)
+   end subroutine get_int
+
+   subroutine get_logical(key, x)
+      character(len=*), intent(in) :: key
+      logical, intent(inout) :: x
+      character(len=256) :: v
+      integer :: ios
+      if (.not. find(key, v)) return
+      read(v, *, iostat=ios) x
+      if (ios /= 0) call bad(key, v)
+   end subroutine get_logical
+
+   subroutine get_string(key, x)
+      character(len=*), intent(in) :: key
+      character(len=*), intent(inout) :: x
+      character(len=256) :: v
+      integer :: n
+      if (.not. find(key, v)) return
+      v = adjustl(v)
+      n = len_trim(v)
+      if (n >= 2 .and. (v(1:1) == "'" .or. v(1:1) == '"')) v = v(2:n - 1)
+      x = trim(v)
+   end subroutine get_string
+
+   subroutine bad(key, v)
+      character(len=*), intent(in) :: key, v
+      write(*, '(4a)') ' >>> ERROR: cannot read ', trim(key), ' = ', trim(v)
+      stop 1
+   end subroutine bad
+
+   !> Position the unit just after the card line `name`; false if absent.
+   logical function seek_card(u, name)
+      integer, intent(in) :: u
+      character(len=*), intent(in) :: name
+      character(len=256) :: line
+      integer :: ios, p
+      rewind(u)
+      seek_card = .false.
+      do
+         read(u, '(a)', iostat=ios) line
+         if (ios /= 0) return
+         p = index(line, '!')
+         if (p > 0) line = line(1:p - 1)
+         if (trim(adjustl(lower(line))) == lower(name)) then
+            seek_card = .true.
+            return
+         end if
+      end do
+   end function seek_card
+
+   !> Integers of one line, "a b c" or ranges "a-b" (SURFACE_ATOMS).
+   subroutine parse_index_list(line, out, n)
+      character(len=*), intent(in) :: line
+      integer, allocatable, intent(out) :: out(:)
+      integer, intent(out) :: n
+      integer :: tmp(4096), i, start, dash, a, b, k, ios
+      character(len=64) :: tok
+      character(len=len(line)) :: s
+      s = line
+      i = index(s, '!')
+      if (i > 0) s = s(1:i - 1)
+      do i = 1, len_trim(s)
+         if (s(i:i) == ',') s(i:i) = ' '
+      end do
+      n = 0
+      start = 1
+      do i = 1, len_trim(s) + 1
+         if (i <= len_trim(s)) then
+            if (s(i:i) /= ' ') cycle
+         end if
+         tok = s(start:i - 1)
+         start = i + 1
+         if (len_trim(tok) == 0) cycle
+         dash = index(tok, '-')
+         if (dash > 1) then
+            read(tok(1:dash - 1), *, iostat=ios) a
+            read(tok(dash + 1:), *, iostat=ios) b
+         else
+            read(tok, *, iostat=ios) a
+            b = a
+         end if
+         do k = a, b
+            n = n + 1
+            tmp(n) = k
+         end do
+      end do
+      allocate(out(n))
+      out = tmp(1:n)
+   end subroutine parse_index_list
+
+   subroutine read_wt_in(fname)
+      character(len=*), intent(in) :: fname
+      real(dp) :: eta_arc, e_arc, lat(3, 3), r1(3), r2(3), s(3, 3), urot(3, 3), ra2(2), rb2(2)
+      real(dp) :: vol, kstart(2), kend(2), kp(2, 64), ke(2, 64), k1(2), k2(2), t1, temp
+      integer :: u, ios, i, j, ia, io, natoms, nk2lines, ntop_atoms, nbot_atoms, nproj_tot
+      integer, allocatable :: nprojs(:), orbitals_start(:), topatoms(:), botatoms(:)
+      character(len=256) :: line
+      character(len=16) :: name1, name2
+      logical :: have_surface
+
+      call scan_namelists(fname)
+      call get_string('Hrfile', Hrfile)
+      call get_string('Particle', Particle)
+      call get_logical('SlabSS_calc', SlabSS_calc)
+      call get_logical('SlabArc_calc', SlabArc_calc)
+      call get_logical('SlabSpintexture_calc', SlabSpintexture_calc)
+      call get_logical('SlabQPI_kplane_calc', SlabQPI_kplane_calc)
+      call get_logical('LOTO_correction', LOTO_correction)
+      call get_int('SOC', Soc)
+      call get_real('E_FERMI', E_fermi)
+      ! src/readinput.f90:588-599, 665-666, 721-724
+      eta_arc = -999d0
+      e_arc = -999d0
+      call get_real('Eta_Arc', eta_arc)
+      call get_real('E_arc', e_arc)
+      call get_real('Fermi_broadening', Fermi_broadening)
+      call get_real('iso_energy', iso_energy)
+      call get_int('OmegaNum', OmegaNum)
+      call get_real('OmegaMin', OmegaMin)
+      call get_real('OmegaMax', OmegaMax)
+      call get_int('Nk1', Nk1)
+      call get_int('Nk2', Nk2)
+      call get_int('NP', Np)
+      if (eta_arc > -998d0) Fermi_broadening = eta_arc
+      if (e_arc > -998d0) iso_energy = e_arc
+      iso_energy = iso_energy*eV2Hartree
+      Fermi_broadening = Fermi_broadening*eV2Hartree
+      OmegaMin = OmegaMin*eV2Hartree
+      OmegaMax = OmegaMax*eV2Hartree
+
+      open(newunit=u, file=fname, status='old', action='read')
+      ! LATTICE: unit line, then a1, a2, a3 as rows
+      if (.not. seek_card(u, 'LATTICE')) stop ' >>> ERROR: wt.in has no LATTICE card'
+      read(u, '(a)') line
+      do i = 1, 3
+         read(u, *) lat(:, i)
+      end do
+      if (index(lower(line), 'bohr') == 0) lat = lat*Angstrom2atomic
+      ! ATOM_POSITIONS / PROJECTORS: orbitals per atom
+      if (.not. seek_card(u, 'ATOM_POSITIONS')) stop ' >>> ERROR: wt.in has no ATOM_POSITIONS card'
+      read(u, *) natoms
+      if (.not. seek_card(u, 'PROJECTORS')) stop ' >>> ERROR: wt.in has no PROJECTORS card'
+      allocate(nprojs(natoms), orbitals_start(natoms + 1))
+      read(u, *) nprojs
+      nproj_tot = sum(nprojs)
+      orbitals_start(1) = 1
+      do i = 1, natoms
+         orbitals_start(i + 1) = orbitals_start(i) + nprojs(i)
+      end do
+      Num_wann = nproj_tot
+      if (Soc > 0) Num_wann = 2*nproj_tot
+      Ndim = Num_wann*Np
+
+      ! SURFACE: R1, R2 in lattice-vector units; only the crystal's own cell
+      s = 0d0
+      s(1, 1) = 1d0
+      s(2, 2) = 1d0
+      s(3, 3) = 1d0
+      have_surface = seek_card(u, 'SURFACE')
+      if (have_surface) then
+         read(u, *) s(1, :)
+         read(u, *) s(2, :)
+      end if
+      if (any(abs(s(1, :) - (/1d0, 0d0, 0d0/)) > 1d-9) .or. any(abs(s(2, :) - (/0d0, 1d0, 0d0/)) > 1d-9)) then
+         write(*, '(a)') ' >>> ERROR: this build supports SURFACE = (1 0 0; 0 1 0) only'
+         stop 1
+      end if
+      r1 = lat(:, 1)
+      r2 = lat(:, 2)
+      ! surface frame and reciprocal vectors (src/readinput.f90:1812-1855)
+      urot(1, :) = r1/sqrt(sum(r1**2))
+      urot(3, 1) = r1(2)*r2(3) - r1(3)*r2(2)
+      urot(3, 2) = r1(3)*r2(1) - r1(1)*r2(3)
+      urot(3, 3) = r1(1)*r2(2) - r1(2)*r2(1)
+      urot(3, :) = urot(3, :)/sqrt(sum(urot(3, :)**2))
+      urot(2, 1) = urot(3, 2)*urot(1, 3) - urot(3, 3)*urot(1, 2)
+      urot(2, 2) = urot(3, 3)*urot(1, 1) - urot(3, 1)*urot(1, 3)
+      urot(2, 3) = urot(3, 1)*urot(1, 2) - urot(3, 2)*urot(1, 1)
+      urot(2, :) = urot(2, :)/sqrt(sum(urot(2, :)**2))
+      ra2(1) = sum(urot(1, :)*r1)
+      ra2(2) = sum(urot(2, :)*r1)
+      rb2(1) = sum(urot(1, :)*r2)
+      rb2(2) = sum(urot(2, :)*r2)
+      vol = abs(ra2(1)*rb2(2) - rb2(1)*ra2(2))
+      if (vol < 1e-6) stop ' >>> ERROR: cell_volume equal zero'
+      Ka2(1) = 2d0*pi/vol*rb2(2)
+      Ka2(2) = -2d0*pi/vol*rb2(1)
+      Kb2(1) = -2d0*pi/vol*ra2(2)
+      Kb2(2) = 2d0*pi/vol*ra2(1)
+
+      ! KPATH_SLAB (src/readinput.f90:2134-2213), Nk = Nk1 points per line
+      knv2 = 0
+      if (seek_card(u, 'KPATH_SLAB')) then
+         read(u, *) nk2lines
+         do i = 1, nk2lines
+            read(u, *) name1, kp(:, i), name2, ke(:, i)
+         end do
+         knv2 = Nk1*nk2lines
+         allocate(k2_path(knv2, 2), k2len(knv2))
+         t1 = 0d0
+         do j = 1, nk2lines
+            do i = 1, Nk1
+               kstart = kp(:, j)
+               kend = ke(:, j)
+               k1 = kstart(1)*Ka2 + kstart(2)*Kb2
+               k2 = kend(1)*Ka2 + kend(2)*Kb2
+               k2_path(i + (j - 1)*Nk1, :) = kstart + (kend - kstart)*(i - 1)/dble(Nk1 - 1)
+               temp = dsqrt((k2(1) - k1(1))**2 + (k2(2) - k1(2))**2)/dble(Nk1 - 1)
+               if (i > 1) t1 = t1 + temp
+               k2len(i + (j - 1)*Nk1) = t1
+            end do
+         end do
+      else if (SlabSS_calc) then
+         stop 'ERROR: please set KPATH_SLAB for slab band structure calculation'
+      end if
+
+      ! KPLANE_SLAB (src/readinput.f90:2216-2262)
+      if (seek_card(u, 'KPLANE_SLAB')) then
+         read(u, *) K2D_start
+         read(u, *) K2D_vec1
+         read(u, *) K2D_vec2
+      else if (SlabArc_calc) then
+         stop 'ERROR: please set KPLANE_SLAB for arc or spintexture calculations'
+      end if
+
+      ! SURFACE_ATOMS -> TopOrbitals / BottomOrbitals (src/readinput.f90:2851-2962)
+      if (seek_card(u, 'SURFACE_ATOMS')) then
+         read(u, '(a)') line
+         call parse_index_list(line, topatoms, ntop_atoms)
+         read(u, '(a)') line
+         call parse_index_list(line, botatoms, nbot_atoms)
+      else
+         ntop_atoms = natoms
+         nbot_atoms = natoms
+         allocate(topatoms(natoms), botatoms(natoms))
+         topatoms = (/(i, i=1, natoms)/)
+         botatoms = topatoms
+      end if
+      close(u)
+      NtopOrbitals = 0
+      do i = 1, ntop_atoms
+         NtopOrbitals = NtopOrbitals + nprojs(topatoms(i))
+      end do
+      if (Soc > 0) NtopOrbitals = NtopOrbitals*2
+      allocate(TopOrbitals(NtopOrbitals))
+      io = 0
+      do i = 1, ntop_atoms
+         ia = topatoms(i)
+         do j = 1, nprojs(ia)
+            io = io + 1
+            TopOrbitals(io) = orbitals_start(ia) + j - 1
+            if (Soc > 0) TopOrbitals(io + NtopOrbitals/2) = orbitals_start(ia) + j - 1 + nproj_tot
+         end do
+      end do
+      NBottomOrbitals = 0
+      do i = 1, nbot_atoms
+         NBottomOrbitals = NBottomOrbitals + nprojs(botatoms(i))
+      end do
+      if (Soc > 0) NBottomOrbitals = NBottomOrbitals*2
+      allocate(BottomOrbitals(NBottomOrbitals))
+      io = 0
+      do i = 1, nbot_atoms
+         ia = botatoms(i)
+         do j = 1, nprojs(ia)
+            io = io + 1
+            BottomOrbitals(io) = orbitals_start(ia) + j - 1
+            if (Soc > 0) BottomOrbitals(io + NBottomOrbitals/2) = orbitals_start(ia) + j - 1 + nproj_tot
+         end do
+      end do
+   end subroutine read_wt_in
+
+   !> Wannier90 *_hr.dat (src/readHmnR.f90:3-305 for the orthogonal basis):
+   !> E_fermi off the R = 0 diagonal, eV -> Hartree; with the crystal's own
+   !> surface cell, HmnR_surfacecell = HmnR and ndegen is applied on use.
+   subroutine read_hr()
+      integer :: u, ios, nw, nr, ir, m, n, i, r(3), mm, nn
+      real(dp) :: re, im
+      integer, allocatable :: ndeg(:)
+      character(len=256) :: line
+      open(newunit=u, file=trim(Hrfile), status='old', action='read', iostat=ios)
+      if (ios /= 0) then
+         write(*, '(3a)') ' >>> ERROR: cannot open the Hrfile ', trim(Hrfile)
+         stop 1
+      end if
+      read(u, '(a)') line
+      read(u, *) nw
+      read(u, *) nr
+      if (nw /= Num_wann) then
+         write(*, '(a, i0, a, i0)') ' >>> ERROR: Hrfile num_wann ', nw, ' but PROJECTORS give ', Num_wann
+         stop 1
+      end if
+      allocate(ndeg(nr))
+      read(u, *) ndeg
+      nrpts_surfacecell = nr
+      allocate(HmnR_surfacecell(nw, nw, nr), irvec_surfacecell(3, nr), ndegen_surfacecell(nr))
+      HmnR_surfacecell = (0d0, 0d0)
+      do ir = 1, nr
+         do n = 1, nw
+            do m = 1, nw
+               read(u, *) r, mm, nn, re, im
+               HmnR_surfacecell(mm, nn, ir) = dcmplx(re, im)
+            end do
+         end do
+         irvec_surfacecell(:, ir) = r
+      end do
+      close(u)
+      do ir = 1, nr
+         if (all(irvec_surfacecell(:, ir) == 0)) then
+            do i = 1, nw
+               HmnR_surfacecell(i, i, ir) = HmnR_surfacecell(i, i, ir) - E_fermi
+            end do
+         end if
+      end do
+      if (index(Particle, 'phonon') /= 0) then
+         HmnR_surfacecell = HmnR_surfacecell*eV2Hartree*eV2Hartree
+      else
+         HmnR_surfacecell = HmnR_surfacecell*eV2Hartree
+      end if
+      ndegen_surfacecell = ndeg
+   end subroutine read_hr
+
+end module wt_surf_input
+
+
+program wt_surf
+   use para
+   use wt_surf_input
+   use wtsurf_gpu
+   implicit none
+   integer :: c0, c1, rate
+   open(unit=stdout, file='WT.out')
+   write(stdout, '(a)') ' wt_surf: WannierTools surface Green''s function path on the GPU (libwtsurf)'
+   call read_wt_in('wt.in')
+   call read_hr()
+   write(stdout, '(a, i0, a, i0, a, i0)') ' Num_wann ', Num_wann, '  Np ', Np, '  Ndim ', Ndim
+   if (SlabSS_calc) then
+      write(stdout, *) ' '
+      write(stdout, *) 'Start to calculate surface state'
+      call system_clock(c0, rate)
+      call surfstat
+      call system_clock(c1)
+      write(stdout, '(a, f16.3, a)') ' Time cost for SlabSS_calc is about ', dble(c1 - c0)/rate, ' s'
+   end if
+   if (SlabArc_calc) then
+      write(stdout, *) ' '
+      write(stdout, *) 'Start to calculate fermi arc'
+      call system_clock(c0, rate)
+      call SurfaceDOSkk
+      call system_clock(c1)
+      write(stdout, '(a, f16.3, a)') ' Time cost for SlabArc is about ', dble(c1 - c0)/rate, ' s'
+   end if
+   call wtsurf_gpu_finalize()
+   write(stdout, *) 'Congratulations! you finished the calculation.'
+   close(stdout)
+end program wt_surf

@@ -1,0 +1,117 @@
+! iso_c_binding interface to libwtsurf (include/wtsurf.h).
+!
+! Every routine is the C entry point of the same name; arrays are passed as
+! the Fortran arrays themselves (column-major complex(c_double_complex)
+! matrices, 1-based orbital lists), which is the layout the C ABI expects.
+module wtsurf_cabi
+   use iso_c_binding
+   implicit none
+
+   integer(c_int), parameter :: WTSURF_FP64 = 0
+   integer(c_int), parameter :: WTSURF_FP32_REFINE = 1
+   integer(c_int), parameter :: WTSURF_STATUS_SINGULAR = 1
+   integer(c_int), parameter :: WTSURF_STATUS_NOT_CONVERGED = 2
+
+   interface
+      integer(c_int) function wtsurf_ctx_create(device, ctx) bind(C, name='wtsurf_ctx_create')
+         import :: c_int, c_ptr
+         integer(c_int), value :: device
+         type(c_ptr), intent(out) :: ctx
+      end function
+
+      subroutine wtsurf_ctx_destroy(ctx) bind(C, name='wtsurf_ctx_destroy')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end subroutine
+
+      type(c_ptr) function wtsurf_last_error(ctx) bind(C, name='wtsurf_last_error')
+         import :: c_ptr
+         type(c_ptr), value :: ctx
+      end function
+
+      integer(c_int) function wtsurf_set_model(ctx, nw, np, nr, ijmax, irvec, hr) &
+            bind(C, name='wtsurf_set_model')
+         import :: c_int, c_ptr, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nw, np, nr, ijmax
+         integer(c_int), intent(in) :: irvec(3, *)
+         complex(c_double_complex), intent(in) :: hr(*)
+      end function
+
+      integer(c_int) function wtsurf_set_stop_rule(ctx, itermax, accuracy) &
+            bind(C, name='wtsurf_set_stop_rule')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: itermax
+         real(c_double), value :: accuracy
+      end function
+
+      integer(c_int) function wtsurf_set_orbitals(ctx, ntop, top, nbot, bot) &
+            bind(C, name='wtsurf_set_orbitals')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: ntop, nbot
+         integer(c_int), intent(in) :: top(*), bot(*)
+      end function
+
+      integer(c_int) function wtsurf_ham_qlayer2qlayer(ctx, nk, kpts, h00, h01) &
+            bind(C, name='wtsurf_ham_qlayer2qlayer')
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nk
+         real(c_double), intent(in) :: kpts(2, *)
+         complex(c_double_complex), intent(out) :: h00(*), h01(*)
+      end function
+
+      integer(c_int) function wtsurf_surfgreen_1985(ctx, n, omega, eta, h00, h01, gll, grr, gb, iters) &
+            bind(C, name='wtsurf_surfgreen_1985')
+         import :: c_int, c_ptr, c_double, c_double_complex
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: n
+         real(c_double), value :: omega, eta
+         complex(c_double_complex), intent(in) :: h00(*), h01(*)
+         complex(c_double_complex), intent(out) :: gll(*), grr(*), gb(*)
+         integer(c_int), intent(out) :: iters
+      end function
+
+      integer(c_int) function wtsurf_spectral_grid(ctx, nk, kpts, nomega, omega, eta, divisor, &
+            precision, dos_l, dos_r, dos_bulk, iters, status) bind(C, name='wtsurf_spectral_grid')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nk, nomega, precision
+         real(c_double), intent(in) :: kpts(2, *), omega(*)
+         real(c_double), value :: eta, divisor
+         real(c_double), intent(out) :: dos_l(*), dos_r(*), dos_bulk(*)
+         integer(c_int), intent(out) :: iters(*), status(*)
+      end function
+
+      integer(c_int) function wtsurf_max_ndim() bind(C, name='wtsurf_max_ndim')
+         import :: c_int
+      end function
+   end interface
+
+contains
+
+   !> Message of the last failed call on ctx, as a Fortran string.
+   function wtsurf_error_message(ctx) result(msg)
+      type(c_ptr), intent(in) :: ctx
+      character(len=:), allocatable :: msg
+      character(kind=c_char), pointer :: p(:)
+      type(c_ptr) :: cp
+      integer :: n
+      cp = wtsurf_last_error(ctx)
+      if (.not. c_associated(cp)) then
+         msg = ''
+         return
+      end if
+      call c_f_pointer(cp, p, [4096])
+      n = 0
+      do while (n < 4096)
+         if (p(n + 1) == c_null_char) exit
+         n = n + 1
+      end do
+      allocate(character(len=n) :: msg)
+      msg = transfer(p(1:n), msg)
+   end function wtsurf_error_message
+
+end module wtsurf_cabi

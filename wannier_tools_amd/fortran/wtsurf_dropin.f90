@@ -1,0 +1,341 @@
+! GPU drop-in for WannierTools' surface Green's function path.
+!
+! External subroutines with the reference's names and signatures, reading the
+! same module-`para` state and writing the same files, computed on an MI355X
+! through libwtsurf (C ABI in include/wtsurf.h, bound in wtsurf_cabi.f90):
+!
+!   ham_qlayer2qlayer(k, H00new, H01new)      src/ham_qlayer2qlayer.f90:1-102
+!   surfgreen_1985(omega, GLL, GRR, GB, H00, H01, ones, eta_broadening)
+!                                             src/surfgreen.f90:7-137
+!   surfstat                                  src/surfstat.f90:1-605
+!   SurfaceDOSkk                              src/fermiarc.f90:1-494
+!
+! ham_qlayer2qlayer and surfgreen_1985 are per-call, for callers that loop
+! themselves.  surfstat and SurfaceDOSkk submit their whole (k, omega) grid to
+! the device in one call.  This file compiles against the reference's
+! module.f90 (para, wmpi) or against para_subset.f90.
+!
+! Not covered here: the spin-resolved traces (spindos.dat_*, QPI and spin
+! texture outputs) and the phonon LO-TO branch; the drivers stop with a
+! message where the reference would need them.
+module wtsurf_gpu
+   use iso_c_binding
+   use wtsurf_cabi
+   implicit none
+   private
+   public :: wtsurf_gpu_ctx, wtsurf_gpu_upload_model, wtsurf_gpu_check, wtsurf_gpu_finalize
+   public :: wtsurf_gpu_precision
+
+   type(c_ptr), save :: ctx = c_null_ptr
+   logical, save :: model_on_device = .false.
+   !> FP64 (the reference's arithmetic) unless the caller selects otherwise
+   integer(c_int), save :: wtsurf_gpu_precision = WTSURF_FP64
+
+contains
+
+   !> The context on HIP device WTSURF_DEVICE (default 0), created on first use.
+   function wtsurf_gpu_ctx() result(c)
+      type(c_ptr) :: c
+      integer :: dev, stat
+      character(len=16) :: val
+      if (.not. c_associated(ctx)) then
+         dev = 0
+         call get_environment_variable('WTSURF_DEVICE', val, status=stat)
+         if (stat == 0) read(val, *, iostat=stat) dev
+         if (wtsurf_ctx_create(int(dev, c_int), ctx) /= 0) then
+            write(*, '(a, i0)') ' >>> ERROR: no usable HIP device for libwtsurf, device ', dev
+            stop 1
+         end if
+      end if
+      c = ctx
+   end function wtsurf_gpu_ctx
+
+   subroutine wtsurf_gpu_check(rc, what)
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: what
+      if (rc /= 0) then
+         write(*, '(4a)') ' >>> ERROR in ', what, ': ', wtsurf_error_message(ctx)
+         stop 1
+      end if
+   end subroutine wtsurf_gpu_check
+
+   !> Push HmnR_surfacecell / irvec_surfacecell (divided by ndegen_surfacecell,
+   !> as ham_qlayer2qlayer applies it, src/ham_qlayer2qlayer.f90:58-60) and the
+   !> surface orbital lists to the device.  force=.false. keeps a model that is
+   !> already there.
+   subroutine wtsurf_gpu_upload_model(force)
+      use para
+      logical, intent(in) :: force
+      complex(dp), allocatable :: hr(:, :, :)
+      integer :: ir
+      type(c_ptr) :: c
+      if (model_on_device .and. .not. force) return
+      c = wtsurf_gpu_ctx()
+      allocate(hr(Num_wann, Num_wann, nrpts_surfacecell))
+      do ir = 1, nrpts_surfacecell
+         hr(:, :, ir) = HmnR_surfacecell(:, :, ir)/ndegen_surfacecell(ir)
+      end do
+      call wtsurf_gpu_check(wtsurf_set_model(c, int(Num_wann, c_int), int(Np, c_int), &
+         int(nrpts_surfacecell, c_int), int(ijmax, c_int), irvec_surfacecell, hr), 'wtsurf_set_model')
+      deallocate(hr)
+      if (allocated(TopOrbitals) .and. allocated(BottomOrbitals)) then
+         call wtsurf_gpu_check(wtsurf_set_orbitals(c, int(NtopOrbitals, c_int), TopOrbitals, &
+            int(NBottomOrbitals, c_int), BottomOrbitals), 'wtsurf_set_orbitals')
+      end if
+      model_on_device = .true.
+   end subroutine wtsurf_gpu_upload_model
+
+   subroutine wtsurf_gpu_finalize()
+      if (c_associated(ctx)) call wtsurf_ctx_destroy(ctx)
+      ctx = c_null_ptr
+      model_on_device = .false.
+   end subroutine wtsurf_gpu_finalize
+
+end module wtsurf_gpu
+
+
+!> H00(k), H01(k) between principal layers (src/ham_qlayer2qlayer.f90:1-102).
+subroutine ham_qlayer2qlayer(k, H00new, H01new)
+   use para
+   use wtsurf_gpu
+   use wtsurf_cabi
+   implicit none
+   real(Dp), intent(in) :: k(2)
+   complex(Dp), intent(out) :: H00new(Ndim, Ndim)
+   complex(Dp), intent(out) :: H01new(Ndim, Ndim)
+   real(Dp) :: kk(2, 1)
+   call wtsurf_gpu_upload_model(.false.)
+   kk(:, 1) = k
+   call wtsurf_gpu_check(wtsurf_ham_qlayer2qlayer(wtsurf_gpu_ctx(), 1_c_int, kk, H00new, H01new), &
+      'wtsurf_ham_qlayer2qlayer')
+end subroutine ham_qlayer2qlayer
+
+
+!> Lopez-Sancho decimation for one (omega, eta) (src/surfgreen.f90:7-137).
+!> `ones` is accepted for the signature; the identity is implied.
+subroutine surfgreen_1985(omega, GLL, GRR, GB, H00, H01, ones, eta_broadening)
+   use para
+   use wtsurf_gpu
+   use wtsurf_cabi
+   implicit none
+   real(Dp), intent(in) :: omega
+   real(Dp), intent(in) :: eta_broadening
+   complex(Dp), intent(in) :: H00(Ndim, Ndim)
+   complex(Dp), intent(in) :: H01(Ndim, Ndim)
+   complex(Dp), intent(in) :: ones(Ndim, Ndim)
+   complex(Dp), intent(inout) :: GLL(Ndim, Ndim)
+   complex(Dp), intent(inout) :: GRR(Ndim, Ndim)
+   complex(Dp), intent(inout) :: GB(Ndim, Ndim)
+   integer(c_int) :: iters
+   call wtsurf_gpu_check(wtsurf_surfgreen_1985(wtsurf_gpu_ctx(), int(Ndim, c_int), omega, &
+      eta_broadening, H00, H01, GLL, GRR, GB, iters), 'wtsurf_surfgreen_1985')
+end subroutine surfgreen_1985
+
+
+!> Surface spectral function along KPATH_SLAB (src/surfstat.f90:1-605):
+!> eta = 3 (OmegaMax-OmegaMin)/OmegaNum, omega_j = OmegaMin + (j-1)(OmegaMax-OmegaMin)/OmegaNum,
+!> dos_l/dos_r/dos_bulk traces, log and *_only post-processing, and the
+!> files dos.dat_l, dos.dat_r, dos.dat_bulk in the reference's formats.
+subroutine surfstat
+   use wmpi
+   use para
+   use wtsurf_gpu
+   use wtsurf_cabi
+   implicit none
+   integer :: i, j, ikp, doslfile, dosrfile, dosbulkfile
+   real(dp) :: eta_broadening
+   real(dp), allocatable :: omega(:), kpts(:, :)
+   real(dp), allocatable :: dos_l(:, :), dos_r(:, :), dos_bulk(:, :)
+   real(dp), allocatable :: dos_l_only(:, :), dos_r_only(:, :)
+   real(dp), allocatable :: dos_l_raw(:, :), dos_r_raw(:, :), dos_bulk_raw(:, :)
+   integer(c_int), allocatable :: iters(:, :), status(:, :)
+
+   if (index(Particle, 'phonon') /= 0 .and. LOTO_correction) then
+      write(*, '(a)') ' >>> ERROR: the GPU surfstat has no LO-TO corrected phonon Hamiltonian'
+      stop 1
+   end if
+   if (cpuid /= 0) return  ! one process drives the GPU
+
+   allocate(omega(omeganum), kpts(2, knv2))
+   allocate(dos_l(knv2, omeganum), dos_r(knv2, omeganum), dos_bulk(knv2, omeganum))
+   allocate(dos_l_only(knv2, omeganum), dos_r_only(knv2, omeganum))
+   allocate(dos_l_raw(knv2, omeganum), dos_r_raw(knv2, omeganum), dos_bulk_raw(knv2, omeganum))
+   allocate(iters(knv2, omeganum), status(knv2, omeganum))
+
+   eta_broadening = (omegamax - omegamin)/dble(omeganum)*3.0d0
+   do i = 1, omeganum
+      omega(i) = omegamin + (i - 1)*(omegamax - omegamin)/dble(omeganum)
+   end do
+   if (index(Particle, 'phonon') /= 0) omega = omega*omega
+   do ikp = 1, knv2
+      kpts(:, ikp) = k2_path(ikp, :)
+   end do
+
+   call wtsurf_gpu_upload_model(.true.)
+   call wtsurf_gpu_check(wtsurf_spectral_grid(wtsurf_gpu_ctx(), int(knv2, c_int), kpts, &
+      int(omeganum, c_int), omega, eta_broadening, 1.0d0, wtsurf_gpu_precision, &
+      dos_l_raw, dos_r_raw, dos_bulk_raw, iters, status), 'wtsurf_spectral_grid')
+   if (any(iand(status, WTSURF_STATUS_SINGULAR) /= 0)) print *, 'something wrong with zgesv'
+
+   ! src/surfstat.f90:237-247
+   dos_l = log(abs(dos_l_raw))
+   dos_r = log(abs(dos_r_raw))
+   dos_bulk = log(abs(dos_bulk_raw) + eps9)
+   do ikp = 1, knv2
+      do j = 1, omeganum
+         dos_l_only(ikp, j) = dos_l_raw(ikp, j) - dos_bulk_raw(ikp, j)
+         if (dos_l_only(ikp, j) < 0) dos_l_only(ikp, j) = eps9
+         dos_r_only(ikp, j) = dos_r_raw(ikp, j) - dos_bulk_raw(ikp, j)
+         if (dos_r_only(ikp, j) < 0) dos_r_only(ikp, j) = eps9
+      end do
+   end do
+
+   if (index(Particle, 'phonon') /= 0) omega = sqrt(omega)
+
+   outfileindex = outfileindex + 1
+   doslfile = outfileindex
+   outfileindex = outfileindex + 1
+   dosrfile = outfileindex
+   outfileindex = outfileindex + 1
+   dosbulkfile = outfileindex
+   open(unit=doslfile, file='dos.dat_l')
+   open(unit=dosrfile, file='dos.dat_r')
+   open(unit=dosbulkfile, file='dos.dat_bulk')
+   write(doslfile, '("#", a12, 6a17)') ' k(1/Ang)', ' E(eV)', 'dos_l', 'dos_l_only'
+   write(dosrfile, '("#", a12, 6a17)') ' k(1/Ang)', ' E(eV)', 'dos_r', 'dos_r_only'
+   write(dosbulkfile, '("#", a12, 6a17)') ' k(1/Ang)', ' E(eV)', 'dos_bulk'
+   do ikp = 1, knv2
+      do j = 1, omeganum
+         write(doslfile, 2002) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_l(ikp, j), dos_l_only(ikp, j)
+         write(dosrfile, 2002) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_r(ikp, j), dos_r_only(ikp, j)
+         write(dosbulkfile, 2003) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_bulk(ikp, j)
+      end do
+      write(doslfile, *)
+      write(dosrfile, *)
+      write(dosbulkfile, *)
+   end do
+   close(doslfile)
+   close(dosrfile)
+   close(dosbulkfile)
+   write(stdout, *) 'ndim', ndim
+   write(stdout, *) 'knv2,omeganum,eta_broadening', knv2, omeganum, eta_broadening/eV2Hartree
+   write(stdout, *) 'calculate density of state successfully'
+
+2002 format(4(1X, E16.8))
+2003 format(3(1X, E16.8))
+end subroutine surfstat
+
+
+!> Surface spectral function on KPLANE_SLAB at iso_energy (src/fermiarc.f90:1-494,
+!> the arc part): k-plane with Nk rounded up to odd, eta = Fermi_broadening,
+!> traces divided by pi, the max/10 thresholds and arc.dat_l, arc.dat_r,
+!> arc.dat_bulk in the reference's formats.
+subroutine SurfaceDOSkk
+   use wmpi
+   use para
+   use wtsurf_gpu
+   use wtsurf_cabi
+   implicit none
+   integer :: i, j, ikp, nkx, nky, arclfile, arcrfile, arcbulkfile
+   real(dp) :: omega(1), eta_broadening, dos_l_max, dos_r_max
+   integer, allocatable :: ik12(:, :)
+   real(dp), allocatable :: k12(:, :), k12_shape(:, :)
+   real(dp), allocatable :: dl(:), dr(:), db(:)
+   real(dp), allocatable :: dos_l(:, :), dos_r(:, :), dos_bulk(:, :)
+   real(dp), allocatable :: dos_l_only(:, :), dos_r_only(:, :)
+   integer(c_int), allocatable :: iters(:), status(:)
+
+   if (SlabQPI_kplane_calc .or. (Soc > 0 .and. SlabSpintexture_calc)) then
+      write(*, '(a)') ' >>> ERROR: QPI and spin texture are not part of the GPU SurfaceDOSkk'
+      stop 1
+   end if
+   if (index(Particle, 'phonon') /= 0 .and. LOTO_correction) then
+      write(*, '(a)') ' >>> ERROR: the GPU SurfaceDOSkk has no LO-TO corrected phonon Hamiltonian'
+      stop 1
+   end if
+   if (cpuid /= 0) return
+
+   nkx = Nk1
+   nky = Nk2
+   if (mod(Nk1, 2) == 0) nkx = Nk1 + 1
+   if (mod(Nk2, 2) == 0) nky = Nk2 + 1
+   allocate(ik12(2, nkx*nky), k12(2, nkx*nky), k12_shape(2, nkx*nky))
+   allocate(dl(nkx*nky), dr(nkx*nky), db(nkx*nky), iters(nkx*nky), status(nkx*nky))
+   allocate(dos_l(nkx, nky), dos_r(nkx, nky), dos_bulk(nkx, nky))
+   allocate(dos_l_only(nkx, nky), dos_r_only(nkx, nky))
+   ikp = 0
+   do i = 1, nkx
+      do j = 1, nky
+         ikp = ikp + 1
+         ik12(1, ikp) = i
+         ik12(2, ikp) = j
+         k12(:, ikp) = K2D_start + (i - 1)*K2D_vec1/dble(nkx - 1) + (j - 1)*K2D_vec2/dble(nky - 1)
+         k12_shape(:, ikp) = k12(1, ikp)*Ka2 + k12(2, ikp)*Kb2
+      end do
+   end do
+
+   omega(1) = iso_energy
+   eta_broadening = Fermi_broadening
+   if (index(Particle, 'phonon') /= 0) omega = omega*omega
+
+   call wtsurf_gpu_upload_model(.true.)
+   call wtsurf_gpu_check(wtsurf_spectral_grid(wtsurf_gpu_ctx(), int(nkx*nky, c_int), k12, &
+      1_c_int, omega, eta_broadening, pi, wtsurf_gpu_precision, dl, dr, db, iters, status), &
+      'wtsurf_spectral_grid')
+   if (any(iand(status, WTSURF_STATUS_SINGULAR) /= 0)) print *, 'something wrong with zgesv'
+   do ikp = 1, nkx*nky
+      dos_l(ik12(1, ikp), ik12(2, ikp)) = dl(ikp)
+      dos_r(ik12(1, ikp), ik12(2, ikp)) = dr(ikp)
+      dos_bulk(ik12(1, ikp), ik12(2, ikp)) = db(ikp)
+   end do
+
+   ! src/fermiarc.f90:372-397
+   dos_l_max = maxval(dos_l)
+   dos_r_max = maxval(dos_r)
+   do ikp = 1, nkx*nky
+      dos_l_only(ik12(1, ikp), ik12(2, ikp)) = dos_l(ik12(1, ikp), ik12(2, ikp))
+      if (dos_l_only(ik12(1, ikp), ik12(2, ikp)) < dos_l_max/10d0) dos_l_only(ik12(1, ikp), ik12(2, ikp)) = eps9
+      dos_r_only(ik12(1, ikp), ik12(2, ikp)) = dos_r(ik12(1, ikp), ik12(2, ikp))
+      if (dos_r_only(ik12(1, ikp), ik12(2, ikp)) < dos_r_max/10d0) dos_r_only(ik12(1, ikp), ik12(2, ikp)) = eps9
+   end do
+
+   outfileindex = outfileindex + 100
+   arclfile = outfileindex
+   outfileindex = outfileindex + 1
+   arcrfile = outfileindex
+   outfileindex = outfileindex + 3
+   arcbulkfile = outfileindex
+   open(unit=arclfile, file='arc.dat_l')
+   open(unit=arcrfile, file='arc.dat_r')
+   open(unit=arcbulkfile, file='arc.dat_bulk')
+   write(arclfile, '(a)') '# Surface density of states of bottom surface'
+   write(arclfile, '(a)') '# The coordinate of k is redefined according to the SURFACE card'
+   write(arclfile, '(a)') "# x axis is parallel to R1'"
+   write(arclfile, '(a)') "# z axis is parallel to R1'xR2'"
+   write(arclfile, '(a)') "# y axis is parallel to z x x"
+   write(arclfile, '(30a16)') '#kx', 'ky', 'log(dos)'
+   write(arcrfile, '(a)') '# Surface density of states of top surface'
+   write(arcbulkfile, '(a)') '#Density of states of bulk system after projection onto the surface BZ'
+   write(arcrfile, '(a)') '# The coordinate of k is redefined according to the SURFACE card'
+   write(arcrfile, '(a)') "# x axis is parallel to R1'"
+   write(arcrfile, '(a)') "# z axis is parallel to R1'xR2'"
+   write(arcrfile, '(a)') "# y axis is parallel to z x x"
+   write(arcrfile, '(30a16)') '#kx', 'ky', 'log(dos)'
+   write(arcbulkfile, '(a)') '#Density of states of bulk system after projection onto the surface BZ'
+   write(arcbulkfile, '(a)') '# The coordinate of k is redefined according to the SURFACE card'
+   write(arcbulkfile, '(a)') "# x axis is parallel to R1'"
+   write(arcbulkfile, '(a)') "# z axis is parallel to R1'xR2'"
+   write(arcbulkfile, '(a)') "# y axis is parallel to z x x"
+   write(arcbulkfile, '(30a16)') '#kx', 'ky', 'log(dos)'
+   do ikp = 1, nkx*nky
+      write(arclfile, '(30f16.8)') k12_shape(:, ikp)*Angstrom2atomic, log(dos_l(ik12(1, ikp), ik12(2, ikp)))
+      if (mod(ikp, nky) == 0) write(arclfile, *) ' '
+      write(arcrfile, '(30f16.8)') k12_shape(:, ikp)*Angstrom2atomic, log(dos_r(ik12(1, ikp), ik12(2, ikp)))
+      if (mod(ikp, nky) == 0) write(arcrfile, *) ' '
+      write(arcbulkfile, '(30f16.8)') k12_shape(:, ikp)*Angstrom2atomic, log(abs(dos_bulk(ik12(1, ikp), ik12(2, ikp))))
+      if (mod(ikp, nky) == 0) write(arcbulkfile, *) ' '
+   end do
+   close(arclfile)
+   close(arcrfile)
+   close(arcbulkfile)
+end subroutine SurfaceDOSkk

@@ -224,7 +224,7 @@ def write_hr(path: str, model: TBModel, header: str = "written by wannier_tools_
                 for m in range(nw):
                     v = h[m, n]
                     f.write(f"{r[0]:5d}{r[1]:5d}{r[2]:5d}{m + 1:5d}{n + 1:5d}"
-                            f"{v.real:22.16e}{v.imag:22.16e}\n".replace("e", "E"))
+                            f" {v.real:23.16E} {v.imag:23.16E}\n")
 
 
 def read_hr(path: str, e_fermi: float = 0.0) -> TBModel:
