@@ -34,8 +34,8 @@ enum {
     WTSURF_STATUS_NOT_CONVERGED = 2  /* itermax reached (src/surfgreen.f90:86) */
 };
 
-/* Context on HIP device `device` with its own stream.  No reference
- * counterpart (the reference has no device state). */
+/* Context on HIP device `device` with its own two streams (default
+ * priority).  No reference counterpart (the reference has no device state). */
 int wtsurf_ctx_create(int device, wtsurf_ctx **ctx);
 void wtsurf_ctx_destroy(wtsurf_ctx *ctx);
 /* Message of the last failed call on ctx ("" if none; never NULL). */
@@ -89,10 +89,12 @@ int wtsurf_spectral_grid(wtsurf_ctx *ctx, int nk, const double *kpts,
                          int *status);
 
 /* The same computation split so inputs can stay resident in device memory:
- * upload the grid once, run (enqueued on the context stream; returns after
- * the launches are queued), wait, fetch.  wtsurf_grid_timing reports the
- * device time of the last run's Fourier and decimation stages (HIP events
- * on the context stream).  No reference counterpart. */
+ * upload the grid once, run, wait, fetch.  The run enqueues its work on the
+ * context streams in rounds and watches a device counter of finished
+ * instances to know when to stop; it returns once the last rounds are
+ * queued, shortly before the work completes.  wtsurf_grid_timing reports
+ * the device time of the last run's Fourier and decimation stages (HIP
+ * events on the context's first stream).  No reference counterpart. */
 int wtsurf_grid_upload(wtsurf_ctx *ctx, int nk, const double *kpts, int nomega,
                        const double *omega);
 int wtsurf_grid_run_async(wtsurf_ctx *ctx, double eta, double divisor,

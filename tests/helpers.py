@@ -31,6 +31,23 @@ def max_rel(a, b, floor=1e-300):
     return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
 
 
+def fp64_floor(oracle_lib, nw, np_, irvec, hr, kpts, omegas, eta, top, bot, divisor=1.0,
+               draws=3):
+    """FP64 floor of the spectral traces for these inputs: the largest relative
+    change of dos_l, dos_r, dos_bulk when H(R) is perturbed by one ulp with
+    random signs.  Two correct FP64 implementations (different summation
+    orders) differ by about this much; near a band edge with a small eta it
+    reaches 1e-10 at Ndim 16."""
+    base = oracle_lib.spectral(nw, np_, irvec, hr, kpts, omegas, eta, top, bot, divisor=divisor)
+    rng = np.random.default_rng(7)
+    worst = 0.0
+    for _ in range(draws):
+        hp = hr * (1 + 2.2e-16 * rng.choice([-1.0, 1.0], size=hr.shape))
+        pert = oracle_lib.spectral(nw, np_, irvec, hp, kpts, omegas, eta, top, bot, divisor=divisor)
+        worst = max(worst, *(max_rel(pert[q], base[q]) for q in range(3)))
+    return worst
+
+
 def chain_surface_g(z, e0, t):
     """Surface Green's function of a semi-infinite chain (onsite e0, hopping t):
     g = (z - e0 - sqrt((z-e0)^2 - 4t^2)) / (2 t^2) on the decaying branch."""

@@ -1,15 +1,18 @@
 """Parity of the HIP path (through the C ABI) with the reference and the oracle.
 
-Bar: the Fourier sum to 1e-14 relative (device sincos vs libm), the FP64
-decimation to 1e-10 relative on the spectral traces A(k, omega) (dos_l, dos_r,
-dos_bulk) and on single diagonal entries of G within the FP64
-reproducibility floor measured between the oracle and the reference
-(5e-10 at Ndim = 72, eta = 1e-3 eV).
+Bar: the Fourier sum to 1e-14 relative (device sincos vs libm); the FP64
+decimation on the spectral traces A(k, omega) (dos_l, dos_r, dos_bulk) and on
+single diagonal entries of G within a few times the FP64 floor of the case.
+The floor is the recursion's own sensitivity: perturbing H(R) by one ulp
+moves the oracle's traces by ~1e-12 at Ndim = 12 and 16 but by 1.2e-10 at
+Ndim = 72 with eta = 1e-3 eV (test_oracle.py::test_fp64_floor_of_golden_cases
+pins these numbers), so no FP64 implementation can promise a closer match
+there than that.
 """
 import numpy as np
 import pytest
 
-from helpers import chain_bulk_g, chain_surface_g, golden_model, max_rel
+from helpers import chain_bulk_g, chain_surface_g, fp64_floor, golden_model, max_rel
 from wannier_tools_amd import model as M
 from wannier_tools_amd.surface import STATUS_NOT_CONVERGED, STATUS_SINGULAR, WtsurfError
 
@@ -17,7 +20,16 @@ pytestmark = pytest.mark.gpu
 
 CASES = ["ndim16", "np3_ndim12", "ndim72"]
 TOL_DIAG = {"ndim16": 1e-11, "np3_ndim12": 1e-11, "ndim72": 5e-10}
-TOL_TRACE = 1e-10
+TOL_TRACE = {"ndim16": 1e-11, "np3_ndim12": 1e-11, "ndim72": 5e-10}
+# Traces against the oracle: within FLOOR_X times the FP64 floor of the same
+# inputs (helpers.fp64_floor), and never required below 1e-12.
+FLOOR_X = 5.0
+
+
+def _traces_within_floor(mine, ref, floor):
+    errs = [max_rel(m, r) for m, r in zip(mine, ref)]
+    _say("  max rel err vs oracle:", errs, "fp64 floor:", floor)
+    assert max(errs) < max(FLOOR_X * floor, 1e-12)
 
 
 def _say(*a):
@@ -61,9 +73,10 @@ def test_spectral_grid_vs_reference_golden(gpu, golden, oracle_lib, name):
     nw, np_, irvec, hr = golden_model(g, name)
     gpu.set_model_arrays(nw, np_, irvec, hr)
     dl, dr, db, its, st = gpu.spectral_grid(g["kpts"], g["omegas"], float(g["eta"]))
-    assert max_rel(dl, g["dos_l"]) < TOL_TRACE
-    assert max_rel(dr, g["dos_r"]) < TOL_TRACE
-    assert max_rel(db, g["dos_bulk"]) < TOL_TRACE
+    tol = TOL_TRACE[name]
+    errs = [max_rel(dl, g["dos_l"]), max_rel(dr, g["dos_r"]), max_rel(db, g["dos_bulk"])]
+    _say("  max rel err dos_l, dos_r, dos_bulk vs reference:", errs)
+    assert max(errs) < tol
     assert np.all(st == 0)
     top = M.default_top_orbitals(nw)
     *_, oits = oracle_lib.spectral(nw, np_, irvec, hr, g["kpts"], g["omegas"], float(g["eta"]), top, top)
@@ -82,7 +95,8 @@ def test_spectral_grid_ndim16_subgrid_vs_oracle(gpu, oracle_lib):
     dl, dr, db, its, st = gpu.spectral_grid(kp, om, eta)
     top = M.default_top_orbitals(8)
     odl, odr, odb, oits = oracle_lib.spectral(8, 2, irvec, hr, kp, om, eta, top, top)
-    assert max_rel(dl, odl) < TOL_TRACE and max_rel(dr, odr) < TOL_TRACE and max_rel(db, odb) < TOL_TRACE
+    floor = fp64_floor(oracle_lib, 8, 2, irvec, hr, kp, om, eta, top, top)
+    _traces_within_floor((dl, dr, db), (odl, odr, odb), floor)
     assert np.mean(its == oits) > 0.99
 
 
@@ -105,7 +119,8 @@ def test_uneven_orders_vs_oracle(gpu, oracle_lib, nw, np_):
     dl, dr, db, its, st = gpu.spectral_grid(kp, om, eta, divisor=M.REF_PI)
     odl, odr, odb, oits = oracle_lib.spectral(nw, np_, irvec, hr, kp, om, eta, top, bot,
                                               divisor=M.REF_PI)
-    assert max_rel(dl, odl) < TOL_TRACE and max_rel(dr, odr) < TOL_TRACE and max_rel(db, odb) < TOL_TRACE
+    floor = fp64_floor(oracle_lib, nw, np_, irvec, hr, kp, om, eta, top, bot, divisor=M.REF_PI)
+    _traces_within_floor((dl, dr, db), (odl, odr, odb), floor)
 
 
 def test_chain_closed_form_on_device(gpu):
@@ -172,7 +187,8 @@ def test_new_model_resets_orbital_lists(gpu, oracle_lib):
     dl, dr, db, its, st = gpu.spectral_grid(kp, om, eta)
     top = M.default_top_orbitals(4)
     odl, odr, odb, _ = oracle_lib.spectral(4, 3, irvec, hr, kp, om, eta, top, top)
-    assert max_rel(dr, odr) < TOL_TRACE and max_rel(dl, odl) < TOL_TRACE
+    floor = fp64_floor(oracle_lib, 4, 3, irvec, hr, kp, om, eta, top, top)
+    _traces_within_floor((dl, dr, db), (odl, odr, odb), floor)
 
 
 def test_invalid_arguments_raise(gpu):

@@ -67,6 +67,28 @@ def test_spectral_traces_match_reference_golden(oracle_lib, golden, name):
     assert np.all(its >= 1)
 
 
+@pytest.mark.parametrize("name,lo,hi", [("ndim16", 1e-14, 1e-11), ("np3_ndim12", 1e-14, 1e-11),
+                                        ("ndim72", 3e-11, 5e-10)])
+def test_fp64_floor_of_golden_cases(oracle_lib, golden, name, lo, hi):
+    """The recursion's own FP64 sensitivity, which bounds how closely any FP64
+    implementation can match another: H(R) perturbed by one ulp (random signs,
+    four draws) moves the traces by ~1e-12 at Ndim 12 and 16 but by ~1e-10 at
+    Ndim 72 with eta = 1e-3 eV.  The GPU parity tolerances are a few times
+    these floors."""
+    g = golden(name)
+    nw, np_, irvec, hr = golden_model(g, name)
+    top = M.default_top_orbitals(nw)
+    args = (g["kpts"], g["omegas"], float(g["eta"]), top, top)
+    base = oracle_lib.spectral(nw, np_, irvec, hr, *args)
+    rng = np.random.default_rng(1)
+    worst = 0.0
+    for _ in range(4):
+        hp = hr * (1 + 2.2e-16 * rng.choice([-1.0, 1.0], size=hr.shape))
+        pert = oracle_lib.spectral(nw, np_, irvec, hp, *args)
+        worst = max(worst, *(max_rel(pert[q], base[q]) for q in range(3)))
+    assert lo < worst < hi
+
+
 def test_fourier_single_r_closed_form(oracle_lib):
     """One R = (2, -1, 1) hopping: H01 block (1,1) for Np = 1 is
     H(R) e^{i 2 pi (2 k1 - k2)} with the reference's truncated pi."""

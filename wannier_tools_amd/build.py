@@ -23,6 +23,10 @@ SOURCES = ["decimate.hip", "fourier.hip", "wtsurf_cabi.hip"]
 HIPFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA results in ordinary VGPRs: the decimation kernels budget registers
+    # so that two kernels share a CU, and accumulation registers would count
+    # on top of that budget
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wall", "-Wno-unused-result",
 ]
 

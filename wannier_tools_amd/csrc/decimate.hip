@@ -1,22 +1,37 @@
 // Batched Lopez-Sancho decimation on gfx950, FP64 and FP32-with-FP64-accumulation.
 //
 // Restates surfgreen_1985 (src/surfgreen.f90:74-134) for many (k, omega)
-// instances in one persistent launch.  One workgroup (8 waves) owns one
-// instance at a time and pulls the next instance id from a device counter
-// when its decimation converges, so uneven iteration counts do not idle the
-// chip.  Per iteration:
+// instances.  Per iteration:
 //
 //   g = (z - eps_i)^-1                 blocked Gauss-Jordan in LDS (MFMA updates)
-//   [m1 ; m2]     = [alpha ; beta] g   dual GEMM, g read straight from LDS
-//   [m1 b ; m2 b] -> eps_i, eps_s += m1 b ; beta'  = m2 b    (b = beta)
-//   [m2 a ; m1 a] -> eps_i, eps_t += m2 a ; alpha' = m1 a    (a = alpha)
+//   [m1 ; m2]     = [alpha ; beta] g   dual GEMM
+//   [m1 b ; m2 b] -> eps_s += m1 b ; beta'  = m2 b    (b = beta)
+//   [m2 a ; m1 a] -> eps_t += m2 a ; alpha' = m1 a    (a = alpha)
 //   stop when sum_ij |alpha'_ij| <= accuracy
 //
-// the reference's 1 inverse + 6 GEMMs, grouped into three products that
-// each stage one shared operand through LDS.  R is the working precision of
-// the inverse and the GEMMs: double reproduces the reference; float runs
-// them on the FP32 MFMA while the eps accumulators, the convergence sum, the
-// three final inverses and the traces stay in FP64.
+// the reference's 1 inverse + 6 GEMMs, grouped into three products that each
+// stage one shared operand through LDS.  The reference adds both products to
+// eps_i as well; since every iteration adds m1 b to eps_s, m2 a to eps_t and
+// both to eps_i, starting from H00 for all three, eps_i = eps_s + eps_t - H00
+// at every step, and it is formed that way where it is read instead of being
+// stored (a third fewer FP64 read-modify-writes per iteration; the sum adds
+// two roundings of eps, it does not accumulate them).  R is the working precision of the
+// inverse and the GEMMs: double reproduces the reference; float runs them on
+// the FP32 MFMA while the eps accumulators, the convergence sum, the three
+// final inverses and the traces stay in FP64.
+//
+// Schedule.  The Gauss-Jordan inverse is a chain of pivot steps run by one
+// wave: its time is set by latency, not arithmetic, and it holds the whole
+// matrix in LDS (100 KB at order 80).  The GEMMs are arithmetic-bound and
+// need only a small LDS stage.  The two halves therefore run as two kernels
+// sized to share every CU:
+//   gj_round_kernel    one inverse per instance slot      (4 waves, large LDS)
+//   gemm_round_kernel  the three dual GEMMs per slot      (8 waves, small LDS)
+// Instance slots come in two sets, one per stream; each stream alternates its
+// set between the two kernels (a "round" is one launch of each), so one set's
+// pivot chains run beside the other set's GEMMs.  A slot keeps its decimation
+// state in global memory between rounds and takes the next (k, omega)
+// instance from a device counter once its previous one is written out.
 #include "wtsurf_device.h"
 #include "wtsurf_internal.h"
 
@@ -25,23 +40,36 @@ namespace wtsurf {
 template <typename T>
 constexpr T cmax(T a, T b) { return a > b ? a : b; }
 
-// LDS layout of one workgroup, in bytes.  Regions reuse the same memory in
-// successive phases:
-//   Gauss-Jordan:       [ S = matrix, NP x LD | T = panel pivot rows ]
-//   GEMM with g in LDS: [ S = g                | A0 | A1 staging     ]
-//   GEMM, B staged:     [ B staging ...        | A0 | A1 staging     ]
-//   final FP64 inverse: [ S64                  | T64                 ]
+constexpr int kGjBlock = 256;  // Gauss-Jordan kernel: 4 waves, one per SIMD
+// One workgroup of each kernel shares a CU: per SIMD one Gauss-Jordan wave
+// (at most 144 VGPRs at order 80) and two GEMM waves of at most 184 VGPRs,
+// 144 + 2 x 184 = 512.  amdgpu_num_vgpr counts half the unified register
+// file on this target, hence 92.
+constexpr int kGemmVgprHalf = 92;
+
+// Slot phases (SlotState::phase); zero-filled state is kEmpty.
+enum : int { kEmpty = 0, kLoop = 1, kGemm = 2, kFinal0 = 3, kFinal1 = 4, kFinal2 = 5, kDone = 6 };
+
+// LDS of the Gauss-Jordan kernel: the matrix S (NP x LD, precision R) and the
+// panel pivot rows T; the final inverses reuse it in FP64.
 template <int Q, typename R>
-struct Lay {
+struct GjLay {
     using C = typename Prec<R>::C;
     static constexpr int NP = 16 * Q, LD = NP + 1;
     static constexpr size_t S = (size_t)NP * LD * sizeof(C);
-    static constexpr size_t T = (size_t)kPanel * NP * sizeof(C);
-    static constexpr size_t A = (size_t)2 * kChunk * NP * sizeof(C);
-    static constexpr size_t B = (size_t)2 * kChunk * (NP + 1) * sizeof(C);
-    static_assert(B <= S, "B staging must fit under the matrix region");
     static constexpr size_t fin = (size_t)NP * LD * 16 + (size_t)kPanel * NP * 16;
-    static constexpr size_t bytes = cmax(cmax(S + T, S + 2 * A), fin);
+    static constexpr size_t bytes = cmax(S + (size_t)kPanel * NP * sizeof(C), fin);
+};
+
+// LDS of the GEMM kernel: one K-chunk of B (rows, padded) and of the two A
+// column panels.
+template <int Q, typename R>
+struct GemmLay {
+    using C = typename Prec<R>::C;
+    static constexpr int NP = 16 * Q;
+    static constexpr size_t B = (size_t)kChunk * (NP + 1) * sizeof(C);
+    static constexpr size_t A = (size_t)kChunk * NP * sizeof(C);
+    static constexpr size_t bytes = B + 2 * A;
 };
 
 template <int Q>
@@ -49,259 +77,305 @@ struct Common {
     double2 diag[16 * Q];
     int pivrow[16 * Q];   // pivot row of elimination step k
     int pivstep[16 * Q];  // elimination step at which row i was the pivot (-1: not yet)
-    double red[kWaves];
     int sing;
-    int inst;
+    SlotState s;          // the slot's state while the kernel runs
 };
 
 // ---------------------------------------------------------------------------
-// Blocked Gauss-Jordan inverse of X = z*1 - E (the first n x n block of the
-// padded FP64 matrix E), in precision R, with partial pivoting on |re|+|im|
-// (the zgetrf rule; ties go to the lowest row).  X lives in LDS as
-// S[col][row].  Columns are processed in panels of kPanel: wave 0 eliminates
-// the panel columns in registers (one or two rows per lane, no workgroup
-// barrier per step; the pivot search is a DPP wave maximum plus a ballot) and
-// records the panel's pivot rows of the other columns in T; then every wave
-// applies the panel's row operations to the other columns as one
-// rank-kPanel MFMA update
+// Blocked Gauss-Jordan inverse in LDS.  The matrix lives as S[col][row]
+// (leading dimension NP + 1); columns are eliminated in panels of kPanel with
+// partial pivoting on |re|+|im| (the zgetrf rule; ties go to the lowest row).
+// Rows are never swapped: step k's pivot is row cm.pivrow[k], and at the end
+// X^-1[i][j] = S[cm.pivstep[j]][cm.pivrow[i]].
+
+// Panel elimination by one wave: columns c0 .. c0+kPanel-1 of S, rows
+// r0 = lane and r1 = lane + 64, held in registers for the whole panel (no
+// barrier per step; the pivot search is a DPP wave maximum plus a ballot).
+// Records the pivots in cm.pivrow / cm.pivstep and writes the panel back.
+// Returns true (every lane) on an exactly zero pivot column.
+template <int Q, typename R>
+__device__ __forceinline__ bool gj_panel(typename Prec<R>::C (*S)[16 * Q + 1], Common<Q> &cm,
+                                         int c0, int n, int lane) {
+    using P = Prec<R>;
+    using C = typename P::C;
+    constexpr int NP = 16 * Q;
+    const int r0 = lane < NP ? lane : 0, r1 = lane + 64;
+    const bool has0 = lane < n;
+    const bool two = NP > 64 && uniform(n > 64 ? 1 : 0);  // second rows exist
+    const bool has1 = two && r1 < n;
+    C a0[kPanel], a1[kPanel];
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        a0[j] = S[c0 + j][r0];
+        if (!has0) a0[j].x = a0[j].y = 0;
+        a1[j].x = a1[j].y = 0;
+        if (has1) a1[j] = S[c0 + j][r1];
+    }
+    bool free0 = has0 && cm.pivstep[r0] < 0;
+    bool free1 = has1 && cm.pivstep[has1 ? r1 : 0] < 0;
+    bool sing = false;
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        const int c = c0 + j;
+        if (c >= n) break;
+        const double v0 = free0 ? (double)(fabs(a0[j].x) + fabs(a0[j].y)) : -1.0;
+        const double v1 = free1 ? (double)(fabs(a1[j].x) + fabs(a1[j].y)) : -1.0;
+        const double best = wave_max_f64(two ? fmax_nocanon(v0, v1) : v0);
+        if (!(best > 0.0)) {  // zero (or NaN) pivot column: singular
+            sing = true;
+            break;
+        }
+        // lowest row attaining the maximum: rows 0..63 before 64..127
+        const unsigned long long m0 = __ballot(v0 == best);
+        const unsigned long long m1 = two ? __ballot(v1 == best) : 0ull;
+        const int slot = uniform(m0 ? 0 : 1);
+        const int pl = uniform(__ffsll((long long)(m0 ? m0 : m1)) - 1);
+        const int p = pl + 64 * slot;
+        // pivot row from lane pl
+        C row[kPanel];
+        if (slot == 0) {
+#pragma unroll
+            for (int jj = 0; jj < kPanel; ++jj) {
+                row[jj].x = P::readlane(a0[jj].x, pl);
+                row[jj].y = P::readlane(a0[jj].y, pl);
+            }
+        } else {
+#pragma unroll
+            for (int jj = 0; jj < kPanel; ++jj) {
+                row[jj].x = P::readlane(a1[jj].x, pl);
+                row[jj].y = P::readlane(a1[jj].y, pl);
+            }
+        }
+        const C r = crecip_fast(row[j]);
+        // eliminate column c from every row with the multiplier m = f r
+        // (f = a[j], r = 1/pivot):  a[jj] = s a[jj] - m row[jj] for jj != j,
+        // a[j] = -m, s = 1.  The pivot row takes m = -r, s = 0, which leaves
+        // r row[jj] (the normalised pivot row) in it and r in entry j.
+        {
+            const bool isp = slot == 0 && lane == pl;
+            C m = cmul(a0[j], r);
+            R s = 1;
+            if (isp) {
+                m.x = -r.x;
+                m.y = -r.y;
+                s = 0;
+            }
+#pragma unroll
+            for (int jj = 0; jj < kPanel; ++jj) {
+                if (jj == j) {
+                    a0[jj].x = -m.x;
+                    a0[jj].y = -m.y;
+                } else {
+                    a0[jj].x = s * a0[jj].x - (m.x * row[jj].x - m.y * row[jj].y);
+                    a0[jj].y = s * a0[jj].y - (m.x * row[jj].y + m.y * row[jj].x);
+                }
+            }
+            free0 = free0 && !isp;
+        }
+        if (two) {
+            const bool isp = slot == 1 && lane == pl;
+            C m = cmul(a1[j], r);
+            R s = 1;
+            if (isp) {
+                m.x = -r.x;
+                m.y = -r.y;
+                s = 0;
+            }
+#pragma unroll
+            for (int jj = 0; jj < kPanel; ++jj) {
+                if (jj == j) {
+                    a1[jj].x = -m.x;
+                    a1[jj].y = -m.y;
+                } else {
+                    a1[jj].x = s * a1[jj].x - (m.x * row[jj].x - m.y * row[jj].y);
+                    a1[jj].y = s * a1[jj].y - (m.x * row[jj].y + m.y * row[jj].x);
+                }
+            }
+            free1 = free1 && !isp;
+        }
+        if (lane == 0) {
+            cm.pivrow[c] = p;
+            cm.pivstep[p] = c;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kPanel; ++j) {
+        if (lane < NP) S[c0 + j][r0] = a0[j];
+        if (has1) S[c0 + j][r1] = a1[j];
+    }
+    return sing;
+}
+
+// T[j][col] = row pivrow[c0 + j] of S outside the panel columns (zero in the
+// panel columns and for steps past the last column), by NT threads.
+template <int Q, typename R, int NT>
+__device__ __forceinline__ void gj_gather_t(typename Prec<R>::C (*S)[16 * Q + 1],
+                                            typename Prec<R>::C (*T)[16 * Q], Common<Q> &cm,
+                                            int c0, int n, int tid) {
+    using C = typename Prec<R>::C;
+    constexpr int NP = 16 * Q;
+    for (int e = tid; e < kPanel * NP; e += NT) {
+        const int j = e / NP, col = e % NP;
+        C tv;
+        tv.x = 0;
+        tv.y = 0;
+        if (c0 + j < n && (col < c0 || col >= c0 + kPanel)) tv = S[col][cm.pivrow[c0 + j]];
+        T[j][col] = tv;
+    }
+}
+
+// Rank-kPanel MFMA update of tile (ti, tj) of S for panel c0, outside the
+// panel columns:  S[:, ~C] = S0[:, ~C] + S[:, C] T, where rows pivoted in
+// this panel start from zero (S0 = 0) since their old content is now in T.
+template <int Q, typename R>
+__device__ __forceinline__ void gj_update_tile(typename Prec<R>::C (*S)[16 * Q + 1],
+                                               typename Prec<R>::C (*T)[16 * Q],
+                                               const Common<Q> &cm, int ti, int tj, int c0,
+                                               int lr, int lq) {
+    using P = Prec<R>;
+    using C = typename P::C;
+    typename P::V4 accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < kPanel / 4; ++ks) {
+        const C a = S[c0 + 4 * ks + lq][16 * ti + lr];
+        const C b = T[4 * ks + lq][16 * tj + lr];
+        accR = P::mfma(a.x, b.x, accR);
+        accR = P::mfma(-a.y, b.y, accR);
+        accI = P::mfma(a.x, b.y, accI);
+        accI = P::mfma(a.y, b.x, accI);
+    }
+    const int col = 16 * tj + lr;
+    if (col < c0 || col >= c0 + kPanel) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 16 * ti + P::out_row(lq, v);
+            const int ps = cm.pivstep[row];
+            const bool pivot_here = ps >= c0 && ps < c0 + kPanel;
+            C o = S[col][row];
+            if (pivot_here) o.x = o.y = 0;
+            o.x += accR[v];
+            o.y += accI[v];
+            S[col][row] = o;
+        }
+    }
+}
+
+// Inverse of X = z*1 - E (the first n x n block of the padded FP64 matrix E,
+// or of E + E2 - H when E2 is given) in precision R, by NT threads.
 //
-//   S[:, ~C] = S0[:, ~C] + S[:, C] T      (rows pivoted in this panel: = S[:, C] T)
-//
-// Rows are never swapped: step k's pivot is row pivrow[k], and at the end
-// X^-1[i][j] = S[pivstep[j]][pivrow[i]].
+// Schedule with one panel of lookahead: after panel p is eliminated and its
+// pivot rows gathered into T, all waves apply p's update to the tile column
+// that holds panel p+1; then wave 0 eliminates panel p+1 while the other
+// waves apply p's update to every other tile column.  Panel p+1's pivot rows
+// are gathered into T only after the barrier that ends p's update.
 //
 // out_mode 1: the diagonal of X^-1 into cm.diag (FP64).
 // out_mode 2: the diagonal and the unpadded n x n X^-1 into `full` (FP64).
-// out_mode 3: X^-1 in place in LDS, S[j][i] = X^-1[i][j], pad zero.
+// out_mode 4: X^-1 padded to NP x NP (pad zero), column-major, precision R,
+//             into `gout`.
 // Returns true if an exactly zero pivot was met; the result is then the
 // identity, as the reference's inv hands back (zgesv skips the solve on a
 // zero pivot and inv copies back the identity right-hand side,
 // src/inverse.f90:44-55).
-template <int Q, typename R>
-__device__ bool gj_inverse(unsigned char *lds, Common<Q> &cm, const double2 *__restrict__ E,
-                           double2 z, int n, int out_mode, double2 *__restrict__ full) {
+template <int Q, typename R, int NT>
+__device__ __forceinline__ bool gj_inverse(unsigned char *lds, Common<Q> &cm,
+                                           const double2 *__restrict__ E,
+                                           const double2 *__restrict__ E2,
+                                           const double2 *__restrict__ H, double2 z, int n,
+                                           int out_mode, double2 *__restrict__ full,
+                                           typename Prec<R>::C *__restrict__ gout) {
     using P = Prec<R>;
     using C = typename P::C;
     constexpr int NP = 16 * Q;
     constexpr int LD = NP + 1;
+    constexpr int NWAVES = NT / kWave;
     static_assert(NP <= 128, "the panel wave holds at most two rows per lane");
-    constexpr int TILES = Q * Q;
-    constexpr int TPW = (TILES + kWaves - 1) / kWaves;
-    const int tid = opaque_tid();
+    static_assert(NWAVES >= 2, "the lookahead needs a panel wave and update waves");
+    const int tid = threadIdx.x;
     const int wave = uniform(tid >> 6), lane = tid & 63;
     const int lr = lane & 15, lq = lane >> 4;
     C(*S)[LD] = reinterpret_cast<C(*)[LD]>(lds);
-    C(*T)[NP] = reinterpret_cast<C(*)[NP]>(lds + Lay<Q, R>::S);
+    C(*T)[NP] = reinterpret_cast<C(*)[NP]>(lds + (size_t)NP * LD * sizeof(C));
 
-    for (int e = tid; e < NP * NP; e += kBlock) {
+    for (int e = tid; e < NP * NP; e += NT) {
         const int i = e % NP, j = e / NP;
         C v;
         v.x = 0;
         v.y = 0;
         if (i < n && j < n) {
-            const double2 x = E[i + j * NP];
+            double2 x = E[i + j * NP];
+            if (E2) {
+                const double2 y = E2[i + j * NP], h = H[i + j * NP];
+                x.x = (x.x + y.x) - h.x;
+                x.y = (x.y + y.y) - h.y;
+            }
             v.x = (R)((i == j ? z.x : 0.0) - x.x);
             v.y = (R)((i == j ? z.y : 0.0) - x.y);
         }
         S[j][i] = v;
     }
-    for (int i = tid; i < NP; i += kBlock) cm.pivstep[i] = -1;
+    for (int i = tid; i < NP; i += NT) cm.pivstep[i] = -1;
     if (tid == 0) cm.sing = 0;
     __syncthreads();
 
     const int npanel = (n + kPanel - 1) / kPanel;
+    if (wave == 0) {
+        const bool sing = gj_panel<Q, R>(S, cm, 0, n, lane);
+        if (lane == 0 && sing) cm.sing = 1;
+    }
+    __syncthreads();
     for (int pb = 0; pb < npanel; ++pb) {
-        const int c0 = pb * kPanel;
-        // ---- panel elimination: wave 0, rows r0 = lane and r1 = lane + 64
-        if (wave == 0) {
-            const int r0 = lane < NP ? lane : 0, r1 = lane + 64;
-            const bool has0 = lane < n;
-            const bool two = NP > 64 && uniform(n > 64 ? 1 : 0);  // second rows exist
-            const bool has1 = two && r1 < n;
-            C a0[kPanel], a1[kPanel];
-#pragma unroll
-            for (int j = 0; j < kPanel; ++j) {
-                a0[j] = S[c0 + j][r0];
-                if (!has0) a0[j].x = a0[j].y = 0;
-                a1[j].x = a1[j].y = 0;
-                if (has1) a1[j] = S[c0 + j][r1];
-            }
-            bool free0 = has0 && cm.pivstep[r0] < 0;
-            bool free1 = has1 && cm.pivstep[has1 ? r1 : 0] < 0;
-            bool sing = false;
-            int jdone = 0;
-#pragma unroll
-            for (int j = 0; j < kPanel; ++j) {
-                const int c = c0 + j;
-                if (c >= n) break;
-                const double v0 = free0 ? (double)(fabs(a0[j].x) + fabs(a0[j].y)) : -1.0;
-                const double v1 = free1 ? (double)(fabs(a1[j].x) + fabs(a1[j].y)) : -1.0;
-                const double best = wave_max_f64(two ? fmax(v0, v1) : v0);
-                if (!(best > 0.0)) {  // zero (or NaN) pivot column: singular
-                    sing = true;
-                    break;
-                }
-                // lowest row attaining the maximum: rows 0..63 before 64..127
-                const unsigned long long m0 = __ballot(v0 == best);
-                const unsigned long long m1 = two ? __ballot(v1 == best) : 0ull;
-                const int slot = uniform(m0 ? 0 : 1);
-                const int pl = uniform(__ffsll((long long)(m0 ? m0 : m1)) - 1);
-                const int p = pl + 64 * slot;
-                // this pivot row's entries outside the panel, for the MFMA update
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int col = lane + 64 * h;
-                    if (col < NP) {
-                        C tv;
-                        tv.x = 0;
-                        tv.y = 0;
-                        if (col < c0 || col >= c0 + kPanel) tv = S[col][p];
-                        T[j][col] = tv;
-                    }
-                }
-                // pivot row from lane pl, normalised: entry j -> 1/pivot, others scaled
-                C rowv[kPanel];
-                if (slot == 0) {
-#pragma unroll
-                    for (int jj = 0; jj < kPanel; ++jj) {
-                        rowv[jj].x = P::readlane(a0[jj].x, pl);
-                        rowv[jj].y = P::readlane(a0[jj].y, pl);
-                    }
-                } else {
-#pragma unroll
-                    for (int jj = 0; jj < kPanel; ++jj) {
-                        rowv[jj].x = P::readlane(a1[jj].x, pl);
-                        rowv[jj].y = P::readlane(a1[jj].y, pl);
-                    }
-                }
-                const C r = crecip_uniform(rowv[j]);
-#pragma unroll
-                for (int jj = 0; jj < kPanel; ++jj) rowv[jj] = (jj == j) ? r : cmul(rowv[jj], r);
-                // eliminate column c from every row (pivot row overwritten below)
-                {
-                    const C f = a0[j];
-                    a0[j].x = a0[j].y = 0;
-#pragma unroll
-                    for (int jj = 0; jj < kPanel; ++jj) {
-                        a0[jj].x -= f.x * rowv[jj].x - f.y * rowv[jj].y;
-                        a0[jj].y -= f.x * rowv[jj].y + f.y * rowv[jj].x;
-                    }
-                }
-                if (two) {
-                    const C f = a1[j];
-                    a1[j].x = a1[j].y = 0;
-#pragma unroll
-                    for (int jj = 0; jj < kPanel; ++jj) {
-                        a1[jj].x -= f.x * rowv[jj].x - f.y * rowv[jj].y;
-                        a1[jj].y -= f.x * rowv[jj].y + f.y * rowv[jj].x;
-                    }
-                }
-                if (lane == pl) {
-                    if (slot == 0) {
-#pragma unroll
-                        for (int jj = 0; jj < kPanel; ++jj) a0[jj] = rowv[jj];
-                        free0 = false;
-                    } else {
-#pragma unroll
-                        for (int jj = 0; jj < kPanel; ++jj) a1[jj] = rowv[jj];
-                        free1 = false;
-                    }
-                }
-                if (lane == 0) {
-                    cm.pivrow[c] = p;
-                    cm.pivstep[p] = c;
-                }
-                jdone = j + 1;
-            }
-            // rows of T past the last column of the matrix stay zero
-            for (int j = jdone; j < kPanel; ++j)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int col = lane + 64 * h;
-                    if (col < NP) {
-                        C tv;
-                        tv.x = 0;
-                        tv.y = 0;
-                        T[j][col] = tv;
-                    }
-                }
-#pragma unroll
-            for (int j = 0; j < kPanel; ++j) {
-                if (lane < NP) S[c0 + j][r0] = a0[j];
-                if (has1) S[c0 + j][r1] = a1[j];
-            }
-            if (lane == 0 && sing) cm.sing = 1;
-        }
-        __syncthreads();
         if (uniform(cm.sing)) break;
-        // ---- rank-kPanel update of every column outside the panel (MFMA)
-        const int cend = c0 + kPanel;
-#pragma unroll
-        for (int s = 0; s < TPW; ++s) {
-            const int t = wave + kWaves * s;
-            if (t < TILES) {
+        const int c0 = pb * kPanel;
+        gj_gather_t<Q, R, NT>(S, T, cm, c0, n, tid);
+        __syncthreads();
+        const bool ahead = pb + 1 < npanel;
+        const int tn = (c0 + kPanel) / 16;  // tile column of panel pb + 1
+        if (ahead) {
+            // the Q tiles of column tn first, shared by all waves
+#pragma unroll 1
+            for (int ti = wave; ti < Q; ti += NWAVES) gj_update_tile<Q, R>(S, T, cm, ti, tn, c0, lr, lq);
+            __syncthreads();
+        }
+        if (ahead && wave == 0) {
+            const bool sing = gj_panel<Q, R>(S, cm, c0 + kPanel, n, lane);
+            if (lane == 0 && sing) cm.sing = 1;
+        } else {
+            // tiles t = tj*Q + ti outside column tn (all of them after the last
+            // panel), dealt round-robin over the update waves
+            const int first = ahead ? 1 : 0, nw = ahead ? NWAVES - 1 : NWAVES;
+#pragma unroll 1
+            for (int t = wave - first; t < Q * Q; t += nw) {
                 const int ti = t % Q, tj = t / Q;
-                typename P::V4 accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
-#pragma unroll
-                for (int ks = 0; ks < kPanel / 4; ++ks) {
-                    const C a = S[c0 + 4 * ks + lq][16 * ti + lr];
-                    const C b = T[4 * ks + lq][16 * tj + lr];
-                    accR = P::mfma(a.x, b.x, accR);
-                    accR = P::mfma(-a.y, b.y, accR);
-                    accI = P::mfma(a.x, b.y, accI);
-                    accI = P::mfma(a.y, b.x, accI);
-                }
-                const int col = 16 * tj + lr;
-                if (col < c0 || col >= cend) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int row = 16 * ti + P::out_row(lq, v);
-                        const int ps = cm.pivstep[row];
-                        const bool pivot_here = ps >= c0 && ps < cend;
-                        C o = S[col][row];
-                        if (pivot_here) o.x = o.y = 0;
-                        o.x += accR[v];
-                        o.y += accI[v];
-                        S[col][row] = o;
-                    }
-                }
+                if (ahead && tj == tn) continue;
+                gj_update_tile<Q, R>(S, T, cm, ti, tj, c0, lr, lq);
             }
         }
         __syncthreads();
     }
 
     const bool singular = uniform(cm.sing) != 0;
-    if (out_mode == 3) {
-        constexpr int RPT = (NP * NP + kBlock - 1) / kBlock;
-        C hold[RPT];
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int e = tid + r * kBlock;
+    if (out_mode == 4) {
+        for (int e = tid; e < NP * NP; e += NT) {
             const int i = e % NP, j = e / NP;
             C v;
             v.x = 0;
             v.y = 0;
-            if (e < NP * NP && i < n && j < n) {
+            if (i < n && j < n) {
                 if (singular)
                     v.x = (i == j) ? 1 : 0;
                 else
                     v = S[cm.pivstep[j]][cm.pivrow[i]];
             }
-            hold[r] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < RPT; ++r) {
-            const int e = tid + r * kBlock;
-            if (e < NP * NP) S[e / NP][e % NP] = hold[r];
+            gout[e] = v;
         }
     } else {
-        for (int i = tid; i < n; i += kBlock) {
+        for (int i = tid; i < n; i += NT) {
             const C v = singular ? C{1, 0} : S[cm.pivstep[i]][cm.pivrow[i]];
             cm.diag[i] = make_double2(v.x, v.y);
         }
         if (out_mode == 2)
-            for (int e = tid; e < n * n; e += kBlock) {
+            for (int e = tid; e < n * n; e += NT) {
                 const int i = e % n, j = e / n;
                 C v;
                 if (singular) {
@@ -318,44 +392,147 @@ __device__ bool gj_inverse(unsigned char *lds, Common<Q> &cm, const double2 *__r
 }
 
 // ---------------------------------------------------------------------------
-// Dual GEMM sharing the right operand:  P0 = A0 * B,  P1 = A1 * B  (padded
-// NP x NP in precision R, K = kdim rounded up to the chunk).  The A column
-// panels (and B, unless B_LDS) are staged through LDS in chunks of kChunk
-// columns, double-buffered with a register prefetch of the next chunk; with
-// B_LDS the right operand is the inverse Gauss-Jordan left in LDS
-// (B[k][j] = S[j][k]).  Tile t in [0, 2*Q*Q) belongs to wave t % 8;
-// accumulators stay in registers for the whole K loop.
-//
-//   MODE 0: O0 = P0, O1 = P1
-//   MODE 1: E1 += P0, E2 += P0 (E1 first, as the reference adds; FP64), O1 = P1
-//   MODE 2: as MODE 1, and returns sum_ij |P1_ij| (FP64) to every thread
-template <int Q, typename R, int MODE, bool B_LDS>
-__device__ double dual_gemm(unsigned char *lds, Common<Q> &cm,
-                            const typename Prec<R>::C *__restrict__ A0,
-                            const typename Prec<R>::C *__restrict__ A1,
-                            const typename Prec<R>::C *__restrict__ B, int kdim,
-                            typename Prec<R>::C *__restrict__ O0, typename Prec<R>::C *__restrict__ O1,
-                            double2 *__restrict__ E1, double2 *__restrict__ E2) {
+// One complex 16x16x4 MFMA step on (re, im) fragments:
+//   accR += a.x b.x - a.y b.y,   accI += a.x b.y + a.y b.x
+// nby = -b.y is passed in so that tiles sharing b negate it once.
+template <typename R>
+__device__ __forceinline__ void cmfma(typename Prec<R>::V4 &accR, typename Prec<R>::V4 &accI,
+                                      typename Prec<R>::C a, typename Prec<R>::C b, R nby) {
+    using P = Prec<R>;
+    accR = P::mfma(a.x, b.x, accR);
+    accR = P::mfma(a.y, nby, accR);
+    accI = P::mfma(a.x, b.y, accI);
+    accI = P::mfma(a.y, b.x, accI);
+}
+
+// Write-back of one output tile (lane layout of the MFMA accumulator) to O,
+// adding sum |P_ij| (FP64) to asum when SUM.
+template <int Q, typename R, bool SUM>
+__device__ __forceinline__ void store_tile(int ti, int tj, int lr, int lq,
+                                           const typename Prec<R>::V4 &accR,
+                                           const typename Prec<R>::V4 &accI,
+                                           typename Prec<R>::C *O, double &asum) {
     using P = Prec<R>;
     using C = typename P::C;
     constexpr int NP = 16 * Q;
-    constexpr int LD = NP + 1;
-    constexpr int TILES = 2 * Q * Q;
-    constexpr int TPW = (TILES + kWaves - 1) / kWaves;
+    const int col = 16 * tj + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int idx = 16 * ti + P::out_row(lq, v) + col * NP;
+        C val;
+        val.x = accR[v];
+        val.y = accI[v];
+        O[idx] = val;
+        if (SUM) asum += hypot((double)val.x, (double)val.y);
+    }
+}
+
+// One FP64 accumulator tile, read ahead of its update.
+struct ETile {
+    double2 e[4];
+};
+
+template <int Q, typename R>
+__device__ __forceinline__ void load_etile(ETile &t, int ti, int tj, int lr, int lq,
+                                           const double2 *__restrict__ E) {
+    constexpr int NP = 16 * Q;
+    const int col = 16 * tj + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) t.e[v] = E[16 * ti + Prec<R>::out_row(lq, v) + col * NP];
+}
+
+template <int Q, typename R>
+__device__ __forceinline__ void add_etile(const ETile &t, int ti, int tj, int lr, int lq,
+                                          const typename Prec<R>::V4 &accR,
+                                          const typename Prec<R>::V4 &accI,
+                                          double2 *__restrict__ E) {
+    constexpr int NP = 16 * Q;
+    const int col = 16 * tj + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        E[16 * ti + Prec<R>::out_row(lq, v) + col * NP] =
+            cadd(t.e[v], make_double2(accR[v], accI[v]));
+}
+
+// E += P over this wave's product-0 tiles (pairs s < PPW, then the single
+// tile when it is product 0).  The pair tiles' reads are all issued before
+// the first update, so the wave waits for one memory round trip for them,
+// not one per tile.
+template <int Q, typename R, int PPW>
+__device__ __forceinline__ void accumulate_p0(const typename Prec<R>::V4 (*pR)[2],
+                                              const typename Prec<R>::V4 (*pI)[2],
+                                              const typename Prec<R>::V4 &qR,
+                                              const typename Prec<R>::V4 &qI, int wave,
+                                              bool single0, int su, int lr, int lq,
+                                              double2 *__restrict__ E) {
+    ETile t[PPW > 0 ? PPW : 1];
+#pragma unroll
+    for (int s = 0; s < PPW; ++s) {
+        const int u = wave + kWaves * s;
+        load_etile<Q, R>(t[s], u % Q, u / Q, lr, lq, E);
+    }
+#pragma unroll
+    for (int s = 0; s < PPW; ++s) {
+        const int u = wave + kWaves * s;
+        add_etile<Q, R>(t[s], u % Q, u / Q, lr, lq, pR[s][0], pI[s][0], E);
+    }
+    if (single0) {
+        ETile t1;
+        load_etile<Q, R>(t1, su % Q, su / Q, lr, lq, E);
+        add_etile<Q, R>(t1, su % Q, su / Q, lr, lq, qR, qI, E);
+    }
+}
+
+// Dual GEMM sharing the right operand:  P0 = A0 * B,  P1 = A1 * B  (padded
+// NP x NP in precision R, K = kdim rounded up to the chunk), by 8 waves.  The
+// A column panels and the B rows are staged through LDS in chunks of kChunk,
+// the next chunk's global loads in flight while the current one is
+// multiplied.  O1 may be the same array as B: every read of B is done before
+// the write-back.
+//
+// Work split: tile (ti, tj) of P0 and the same tile of P1 form a pair that
+// reads one B fragment per step.  Pair u = wave + 8 s (s < PPW) belongs
+// whole to `wave`; the PAIRS % 8 leftover pairs are split into single tiles,
+// one per wave (wave w < 2 REM: pair 8 PPW + w/2, product w % 2).  The pair
+// loop is free of branches so the LDS reads of a chunk can be issued ahead
+// of its MFMAs.  Accumulators stay in registers for the whole K loop.
+//
+//   MODE 0: O0 = P0, O1 = P1
+//   MODE 1: E += P0 (FP64), O1 = P1
+//   MODE 2: as MODE 1, and returns sum_ij |P1_ij| (FP64) to every thread
+template <int Q, typename R, int MODE>
+__device__ __forceinline__ double dual_gemm(unsigned char *lds, double *red,
+                                            const typename Prec<R>::C *__restrict__ A0,
+                                            const typename Prec<R>::C *__restrict__ A1,
+                                            const typename Prec<R>::C *B, int kdim,
+                                            typename Prec<R>::C *O0, typename Prec<R>::C *O1,
+                                            double2 *__restrict__ E) {
+    using P = Prec<R>;
+    using C = typename P::C;
+    using V4 = typename P::V4;
+    constexpr int NP = 16 * Q;
+    constexpr int PAIRS = Q * Q;
+    constexpr int PPW = PAIRS / kWaves;
+    constexpr int REM = PAIRS - PPW * kWaves;
+    static_assert(2 * REM <= kWaves, "leftover tiles must fit one per wave");
     const int tid = opaque_tid();
     const int wave = uniform(tid >> 6), lane = tid & 63;
     const int lr = lane & 15, lq = lane >> 4;
-    C(*Sg)[LD] = reinterpret_cast<C(*)[LD]>(lds);
-    C(*as0)[kChunk][NP] = reinterpret_cast<C(*)[kChunk][NP]>(lds + Lay<Q, R>::S);
-    C(*as1)[kChunk][NP] = reinterpret_cast<C(*)[kChunk][NP]>(lds + Lay<Q, R>::S + Lay<Q, R>::A);
-    C(*bs)[kChunk][NP + 1] = reinterpret_cast<C(*)[kChunk][NP + 1]>(lds);
+    const bool single = wave < 2 * REM;
+    const int su = kWaves * PPW + (wave >> 1), sw = wave & 1;
+    C(*bs)[NP + 1] = reinterpret_cast<C(*)[NP + 1]>(lds);
+    C(*as0)[NP] = reinterpret_cast<C(*)[NP]>(lds + GemmLay<Q, R>::B);
+    C(*as1)[NP] = reinterpret_cast<C(*)[NP]>(lds + GemmLay<Q, R>::B + GemmLay<Q, R>::A);
 
-    typename P::V4 accR[TPW], accI[TPW];
+    V4 pR[PPW > 0 ? PPW : 1][2], pI[PPW > 0 ? PPW : 1][2];
+    V4 qR = {0, 0, 0, 0}, qI = {0, 0, 0, 0};
 #pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        accR[s] = typename P::V4{0, 0, 0, 0};
-        accI[s] = typename P::V4{0, 0, 0, 0};
-    }
+    for (int s = 0; s < PPW; ++s)
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            pR[s][w] = V4{0, 0, 0, 0};
+            pI[s][w] = V4{0, 0, 0, 0};
+        }
 
     const int nchunks = (kdim + kChunk - 1) / kChunk;
     // Staging: element e = tid (+ kBlock) of the B rows and the A0/A1 column
@@ -370,255 +547,316 @@ __device__ double dual_gemm(unsigned char *lds, Common<Q> &cm,
     auto load = [&](int c) {
         const int k0 = c * kChunk;
         if (has0) {
-            if (!B_LDS) b0 = B[bo0 + k0];
+            b0 = B[bo0 + k0];
             x0 = A0[ao0 + k0 * NP];
             y0 = A1[ao0 + k0 * NP];
         }
         if (has1) {
-            if (!B_LDS) b1 = B[bo1 + k0];
+            b1 = B[bo1 + k0];
             x1 = A0[ao1 + k0 * NP];
             y1 = A1[ao1 + k0 * NP];
         }
     };
-    auto stash = [&](int buf) {
+    auto stash = [&]() {
         if (has0) {
-            if (!B_LDS) bs[buf][e0 % kChunk][e0 / kChunk] = b0;
-            as0[buf][e0 / NP][e0 % NP] = x0;
-            as1[buf][e0 / NP][e0 % NP] = y0;
+            bs[e0 % kChunk][e0 / kChunk] = b0;
+            as0[e0 / NP][e0 % NP] = x0;
+            as1[e0 / NP][e0 % NP] = y0;
         }
         if (has1) {
-            if (!B_LDS) bs[buf][e1 % kChunk][e1 / kChunk] = b1;
-            as0[buf][e1 / NP][e1 % NP] = x1;
-            as1[buf][e1 / NP][e1 % NP] = y1;
+            bs[e1 % kChunk][e1 / kChunk] = b1;
+            as0[e1 / NP][e1 % NP] = x1;
+            as1[e1 / NP][e1 % NP] = y1;
         }
     };
 
     load(0);
-    stash(0);
-    __syncthreads();
     for (int c = 0; c < nchunks; ++c) {
-        const int buf = c & 1;
+        stash();
+        __syncthreads();
         if (c + 1 < nchunks) load(c + 1);
 #pragma unroll
-        for (int s = 0; s < TPW; ++s) {
-            const int t = wave + kWaves * s;
-            if (t < TILES) {
-                const int which = t / (Q * Q);
-                const int tt = t - which * Q * Q;
-                const int ti = tt % Q, tj = tt / Q;
-                const C(*as)[NP] = which ? as1[buf] : as0[buf];
+        for (int ks = 0; ks < kChunk / 4; ++ks) {
+            const int kk = 4 * ks + lq;
 #pragma unroll
-                for (int ks = 0; ks < kChunk / 4; ++ks) {
-                    const C a = as[4 * ks + lq][16 * ti + lr];
-                    const C b = B_LDS ? Sg[16 * tj + lr][c * kChunk + 4 * ks + lq]
-                                      : bs[buf][4 * ks + lq][16 * tj + lr];
-                    accR[s] = P::mfma(a.x, b.x, accR[s]);
-                    accR[s] = P::mfma(-a.y, b.y, accR[s]);
-                    accI[s] = P::mfma(a.x, b.y, accI[s]);
-                    accI[s] = P::mfma(a.y, b.x, accI[s]);
-                }
+            for (int s = 0; s < PPW; ++s) {
+                const int u = wave + kWaves * s;
+                const int ti = u % Q, tj = u / Q;
+                const C b = bs[kk][16 * tj + lr];
+                const C x = as0[kk][16 * ti + lr];
+                const C y = as1[kk][16 * ti + lr];
+                const R nby = -b.y;
+                cmfma<R>(pR[s][0], pI[s][0], x, b, nby);
+                cmfma<R>(pR[s][1], pI[s][1], y, b, nby);
             }
         }
-        if (c + 1 < nchunks) stash(buf ^ 1);
+        if (REM > 0 && single) {
+            const int ti = su % Q, tj = su / Q;
+            const C(*as)[NP] = sw ? as1 : as0;
+#pragma unroll
+            for (int ks = 0; ks < kChunk / 4; ++ks) {
+                const int kk = 4 * ks + lq;
+                const C b = bs[kk][16 * tj + lr];
+                const C x = as[kk][16 * ti + lr];
+                cmfma<R>(qR, qI, x, b, -b.y);
+            }
+        }
         __syncthreads();
     }
 
+    // Write-back.  For MODE 1/2 the product-0 tiles read-modify-write the
+    // FP64 accumulator E.
     double asum = 0.0;
+    const bool single0 = REM > 0 && single && sw == 0;
+    if (MODE == 0) {
 #pragma unroll
-    for (int s = 0; s < TPW; ++s) {
-        const int t = wave + kWaves * s;
-        if (t < TILES) {
-            const int which = t / (Q * Q);
-            const int tt = t - which * Q * Q;
-            const int ti = tt % Q, tj = tt / Q;
-            const int col = 16 * tj + lr;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = 16 * ti + P::out_row(lq, v);
-                const int idx = row + col * NP;
-                C val;
-                val.x = accR[s][v];
-                val.y = accI[s][v];
-                if (MODE == 0) {
-                    (which ? O1 : O0)[idx] = val;
-                } else if (which == 0) {
-                    const double2 d = make_double2(val.x, val.y);
-                    E1[idx] = cadd(E1[idx], d);
-                    E2[idx] = cadd(E2[idx], d);
-                } else {
-                    O1[idx] = val;
-                    if (MODE == 2) asum += hypot((double)val.x, (double)val.y);
-                }
-            }
+        for (int s = 0; s < PPW; ++s) {
+            const int u = wave + kWaves * s;
+            store_tile<Q, R, false>(u % Q, u / Q, lr, lq, pR[s][0], pI[s][0], O0, asum);
         }
+        if (single0) store_tile<Q, R, false>(su % Q, su / Q, lr, lq, qR, qI, O0, asum);
+    } else {
+        accumulate_p0<Q, R, PPW>(pR, pI, qR, qI, wave, single0, su, lr, lq, E);
     }
+#pragma unroll
+    for (int s = 0; s < PPW; ++s) {
+        const int u = wave + kWaves * s;
+        store_tile<Q, R, MODE == 2>(u % Q, u / Q, lr, lq, pR[s][1], pI[s][1], O1, asum);
+    }
+    if (REM > 0 && single && sw == 1)
+        store_tile<Q, R, MODE == 2>(su % Q, su / Q, lr, lq, qR, qI, O1, asum);
     if (MODE == 2) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) asum += __shfl_xor(asum, off, 64);
-        if (lane == 0) cm.red[wave] = asum;
+        if (lane == 0) red[wave] = asum;
         __syncthreads();
         double tot = 0.0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) tot += cm.red[w];
+        for (int w = 0; w < kWaves; ++w) tot += red[w];
         return tot;
     }
     return 0.0;
 }
 
-// Per-slot workspace: alpha, alpha', beta, beta', m1, m2 in precision R,
-// then eps_i, eps_s, eps_t in FP64.
+// Per-slot workspace: alpha, beta, m1, m2, g in precision R, then eps_s and
+// eps_t in FP64, each NP x NP column-major.
 template <int Q, typename R>
-constexpr size_t slot_bytes() {
-    constexpr size_t NN = (size_t)(16 * Q) * (16 * Q);
-    return 6 * NN * sizeof(typename Prec<R>::C) + 3 * NN * sizeof(double2);
+struct SlotWork {
+    using C = typename Prec<R>::C;
+    static constexpr size_t NN = (size_t)(16 * Q) * (16 * Q);
+    static constexpr size_t bytes = 5 * NN * sizeof(C) + 2 * NN * sizeof(double2);
+    C *al, *be, *m1, *m2, *g;
+    double2 *es, *et;
+    __device__ SlotWork(void *work, int slot) {
+        unsigned char *base = reinterpret_cast<unsigned char *>(work) + (size_t)slot * bytes;
+        C *c = reinterpret_cast<C *>(base);
+        al = c;
+        be = c + NN;
+        m1 = c + 2 * NN;
+        m2 = c + 3 * NN;
+        g = c + 4 * NN;
+        double2 *d = reinterpret_cast<double2 *>(base + 5 * NN * sizeof(C));
+        es = d;
+        et = d + NN;
+    }
+};
+
+__device__ __forceinline__ void instance_of(const DecimateArgs &a, int t, int &ik, double2 &z) {
+    double om;
+    if (a.inst_k) {
+        ik = a.inst_k[t];
+        om = a.inst_w[t];
+    } else {
+        ik = t / a.nomega;
+        om = a.omega[t - ik * a.nomega];
+    }
+    z = make_double2(om, a.eta ? a.eta[t] : a.eta0);
 }
 
+// One Gauss-Jordan per slot: a slot without an instance takes the next one
+// (eps_s = eps_t = H00, alpha = H01, beta = H01^dagger) and starts
+// its first iteration; kLoop computes g = (z - eps_i)^-1 for the GEMM round;
+// kFinal0..2 compute the diagonals of GLL = (z - eps_s)^-1,
+// GRR = (z - eps_t)^-1, GB = (z - eps_i)^-1 in FP64 and their traces, and the
+// last of them writes the instance's outputs.
 template <int Q, typename R>
-__global__ void __launch_bounds__(kBlock, 1) decimate_kernel(DecimateArgs a) {
+__global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int slot0) {
     using C = typename Prec<R>::C;
     constexpr int NP = 16 * Q;
     constexpr int NN = NP * NP;
-    __shared__ __align__(16) unsigned char lds[Lay<Q, R>::bytes];
+    // The matrix region is dynamic shared memory (GjLay<Q, R>::bytes at
+    // launch): with its size unknown here, the compiler does not take the
+    // one-workgroup-per-CU occupancy it implies as licence to claim every
+    // register, and the kernel's register count stays what it uses, small
+    // enough to leave room for a GEMM workgroup beside it.
+    extern __shared__ __align__(16) unsigned char lds[];
     __shared__ Common<Q> cm;
     const int tid = threadIdx.x;
-    unsigned char *slot = reinterpret_cast<unsigned char *>(a.work) + (size_t)blockIdx.x * slot_bytes<Q, R>();
-    C *wc = reinterpret_cast<C *>(slot);
-    double2 *wd = reinterpret_cast<double2 *>(slot + 6 * (size_t)NN * sizeof(C));
+    const int slot = slot0 + blockIdx.x;
+    SlotState *st = a.state + slot;
+    const SlotWork<Q, R> w(a.work, slot);
     const int n = a.n;
 
-    for (;;) {
-        if (uniform(tid >> 6) == 0) {
-            // wave 0 takes one ticket: lane 0 adds 1, the others add 0
-            const unsigned old = atomicAdd(a.queue, (tid & 63) == 0 ? 1u : 0u);
-            cm.inst = uniform((int)old);
+    if (tid == 0) cm.s = *st;
+    __syncthreads();
+    int phase = uniform(cm.s.phase);
+    if (phase == kGemm || phase == kDone) return;
+    if (phase == kEmpty) {
+        if (tid == 0) cm.s.t = (int)atomicAdd(a.queue, 1u);
+        __syncthreads();
+        const int t = uniform(cm.s.t);
+        if (t >= a.ninst) {
+            if (tid == 0) st->phase = kDone;
+            return;
         }
-        __syncthreads();
-        const int t = uniform(cm.inst);
-        __syncthreads();
-        if (t >= a.ninst) break;
-
         int ik;
-        double om;
-        if (a.inst_k) {
-            ik = a.inst_k[t];
-            om = a.inst_w[t];
-        } else {
-            ik = t / a.nomega;
-            om = a.omega[t - ik * a.nomega];
-        }
-        const double eta = a.eta ? a.eta[t] : a.eta0;
-        const double2 z = make_double2(om, eta);
+        double2 z;
+        instance_of(a, t, ik, z);
         const double2 *H00 = a.hk + (size_t)ik * 2 * NN;
         const double2 *H01 = H00 + NN;
-
-        C *Acur = wc, *Anext = wc + NN, *Bcur = wc + 2 * NN, *Bnext = wc + 3 * NN;
-        C *M1 = wc + 4 * NN, *M2 = wc + 5 * NN;
-        double2 *EI = wd, *ES = wd + NN, *ET = wd + 2 * NN;
-
-        // eps_i = eps_s = eps_t = H00, alpha = H01, beta = H01^dagger
-        for (int e = tid; e < NN; e += kBlock) {
+        for (int e = tid; e < NN; e += kGjBlock) {
             const double2 h = H00[e];
-            EI[e] = h;
-            ES[e] = h;
-            ET[e] = h;
+            w.es[e] = h;
+            w.et[e] = h;
             const double2 h1 = H01[e];
             C av;
             av.x = (R)h1.x;
             av.y = (R)h1.y;
-            Acur[e] = av;
+            w.al[e] = av;
             const int i = e % NP, j = e / NP;
             const double2 ht = H01[j + i * NP];
             C bv;
             bv.x = (R)ht.x;
             bv.y = (R)(-ht.y);
-            Bcur[e] = bv;
-        }
-        __syncthreads();
-
-        int status = 0, it;
-        for (it = 1; it <= a.itermax; ++it) {
-            if (gj_inverse<Q, R>(lds, cm, EI, z, n, 3, nullptr)) status |= 1;
-            dual_gemm<Q, R, 0, true>(lds, cm, Acur, Bcur, nullptr, n, M1, M2, nullptr, nullptr);
-            __syncthreads();
-            dual_gemm<Q, R, 1, false>(lds, cm, M1, M2, Bcur, n, nullptr, Bnext, EI, ES);
-            __syncthreads();
-            const double s = dual_gemm<Q, R, 2, false>(lds, cm, M2, M1, Acur, n, nullptr, Anext, EI, ET);
-            __syncthreads();
-            C *tmp = Acur;
-            Acur = Anext;
-            Anext = tmp;
-            tmp = Bcur;
-            Bcur = Bnext;
-            Bnext = tmp;
-            if (uniform(s <= a.accuracy ? 1 : 0)) break;
-        }
-        if (it > a.itermax) {
-            it = a.itermax;
-            status |= 2;
-        }
-
-        // GLL = (z - eps_s)^-1, GRR = (z - eps_t)^-1, GB = (z - eps_i)^-1, in FP64
-        double dl = 0.0, dr = 0.0, db = 0.0;
-#pragma unroll 1
-        for (int q = 0; q < 3; ++q) {
-            double2 *full = a.gfull ? a.gfull + ((size_t)t * 3 + q) * n * n : nullptr;
-            const double2 *eps = q == 0 ? ES : (q == 1 ? ET : EI);
-            if (gj_inverse<Q, double>(lds, cm, eps, z, n, full ? 2 : 1, full)) status |= 1;
-            if (tid == 0) {
-                if (q == 0) {
-                    for (int i = 0; i < a.ntop; ++i) dl = dl - cm.diag[a.top[i] - 1].y / a.divisor;
-                } else if (q == 1) {
-                    for (int i = 0; i < a.nbot; ++i)
-                        dr = dr - cm.diag[n - a.nw + a.bot[i] - 1].y / a.divisor;
-                } else {
-                    for (int i = 0; i < n; ++i) db = db - cm.diag[i].y / a.divisor;
-                }
-            }
-            __syncthreads();
+            w.be[e] = bv;
         }
         if (tid == 0) {
-            int o = t;
-            if (!a.inst_k) {
-                const int iw = t - ik * a.nomega;
-                o = ik + a.nk * iw;
-            }
-            a.dos_l[o] = dl;
-            a.dos_r[o] = dr;
-            a.dos_bulk[o] = db;
-            if (a.iters) a.iters[o] = it;
-            if (a.status) a.status[o] = status;
+            cm.s.it = 1;
+            cm.s.status = 0;
+            cm.s.dl = 0.0;
+            cm.s.dr = 0.0;
         }
+        phase = kLoop;
+        __syncthreads();
+    }
+    int ik;
+    double2 z;
+    instance_of(a, uniform(cm.s.t), ik, z);
+    const double2 *H00 = a.hk + (size_t)ik * 2 * NN;
+    if (phase == kLoop) {
+        const bool sing =
+            gj_inverse<Q, R, kGjBlock>(lds, cm, w.es, w.et, H00, z, n, 4, nullptr, w.g);
+        if (tid == 0) {
+            if (sing) cm.s.status |= 1;
+            cm.s.phase = kGemm;
+        }
+    } else {
+        const int q = phase - kFinal0;
+        double2 *full = a.gfull ? a.gfull + ((size_t)cm.s.t * 3 + q) * n * n : nullptr;
+        // eps_s, eps_t, eps_i = eps_s + eps_t - H00
+        const double2 *eps = q == 1 ? w.et : w.es;
+        const double2 *eps2 = q == 2 ? w.et : nullptr;
+        const bool sing = gj_inverse<Q, double, kGjBlock>(lds, cm, eps, eps2, H00, z, n,
+                                                          full ? 2 : 1, full, nullptr);
+        if (tid == 0) {
+            if (sing) cm.s.status |= 1;
+            if (q == 0) {
+                double dl = 0.0;
+                for (int i = 0; i < a.ntop; ++i) dl = dl - cm.diag[a.top[i] - 1].y / a.divisor;
+                cm.s.dl = dl;
+                cm.s.phase = kFinal1;
+            } else if (q == 1) {
+                double dr = 0.0;
+                for (int i = 0; i < a.nbot; ++i)
+                    dr = dr - cm.diag[n - a.nw + a.bot[i] - 1].y / a.divisor;
+                cm.s.dr = dr;
+                cm.s.phase = kFinal2;
+            } else {
+                double db = 0.0;
+                for (int i = 0; i < n; ++i) db = db - cm.diag[i].y / a.divisor;
+                const int t = cm.s.t;
+                int o = t;
+                if (!a.inst_k) {
+                    const int iw = t - ik * a.nomega;
+                    o = ik + a.nk * iw;
+                }
+                a.dos_l[o] = cm.s.dl;
+                a.dos_r[o] = cm.s.dr;
+                a.dos_bulk[o] = db;
+                if (a.iters) a.iters[o] = cm.s.it;
+                if (a.status) a.status[o] = cm.s.status;
+                __threadfence();
+                atomicAdd(a.queue + 1, 1u);
+                cm.s.phase = kEmpty;
+            }
+        }
+    }
+    if (tid == 0) *st = cm.s;
+}
+
+// The three dual GEMMs of one iteration for every slot in kGemm, then the
+// stop rule: kFinal0 when sum |alpha'| <= accuracy or at itermax (status bit
+// 1, iterations = itermax), else the next iteration's kLoop.  beta' and
+// alpha' overwrite beta and alpha in place.
+template <int Q, typename R>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(kGemmVgprHalf)))
+gemm_round_kernel(DecimateArgs a, int slot0) {
+    __shared__ __align__(16) unsigned char lds[GemmLay<Q, R>::bytes];
+    __shared__ double red[kWaves];
+    __shared__ int sphase;
+    const int slot = slot0 + blockIdx.x;
+    SlotState *st = a.state + slot;
+    if (threadIdx.x == 0) sphase = st->phase;
+    __syncthreads();
+    if (uniform(sphase) != kGemm) return;
+    const SlotWork<Q, R> w(a.work, slot);
+    const int n = a.n;
+    dual_gemm<Q, R, 0>(lds, red, w.al, w.be, w.g, n, w.m1, w.m2, nullptr);
+    __syncthreads();
+    dual_gemm<Q, R, 1>(lds, red, w.m1, w.m2, w.be, n, nullptr, w.be, w.es);
+    __syncthreads();
+    const double s = dual_gemm<Q, R, 2>(lds, red, w.m2, w.m1, w.al, n, nullptr, w.al, w.et);
+    if (threadIdx.x == 0) {
+        SlotState v = *st;
+        if (s <= a.accuracy) {
+            v.phase = kFinal0;
+        } else if (v.it >= a.itermax) {
+            v.status |= 2;
+            v.phase = kFinal0;
+        } else {
+            v.it += 1;
+            v.phase = kLoop;
+        }
+        *st = v;
     }
 }
 
 constexpr int kMaxQ = 5;
+constexpr int kRoundsPerBatch = 8;
 
 int padded_q(int n) { return (n + 15) / 16; }
 int decimate_max_q() { return kMaxQ; }
 
 template <int Q, typename R>
-static int launch_q(const DecimateArgs &a, int slots, hipStream_t s) {
-    hipLaunchKernelGGL((decimate_kernel<Q, R>), dim3(slots), dim3(kBlock), 0, s, a);
+static int launch_round(const DecimateArgs &a, int per_set, int set, bool gj, hipStream_t s) {
+    constexpr size_t gj_lds = GjLay<Q, R>::bytes;
+    if (gj)
+        hipLaunchKernelGGL((gj_round_kernel<Q, R>), dim3(per_set), dim3(kGjBlock), gj_lds, s, a,
+                           set * per_set);
+    else
+        hipLaunchKernelGGL((gemm_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), 0, s, a,
+                           set * per_set);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int Q, typename R>
-static int occupancy_q() {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, decimate_kernel<Q, R>, kBlock, 0) !=
-        hipSuccess)
-        return 1;
-    return nb > 0 ? nb : 1;
-}
-
 template <typename R>
-static int dispatch_slots(int q) {
+static int dispatch_round(const DecimateArgs &a, int q, int per_set, int set, bool gj, hipStream_t s) {
     switch (q) {
-        case 1: return occupancy_q<1, R>();
-        case 2: return occupancy_q<2, R>();
-        case 3: return occupancy_q<3, R>();
-        case 4: return occupancy_q<4, R>();
-        case 5: return occupancy_q<5, R>();
+        case 1: return launch_round<1, R>(a, per_set, set, gj, s);
+        case 2: return launch_round<2, R>(a, per_set, set, gj, s);
+        case 3: return launch_round<3, R>(a, per_set, set, gj, s);
+        case 4: return launch_round<4, R>(a, per_set, set, gj, s);
+        case 5: return launch_round<5, R>(a, per_set, set, gj, s);
         default: return -1;
     }
 }
@@ -626,39 +864,95 @@ static int dispatch_slots(int q) {
 template <typename R>
 static size_t dispatch_slot_bytes(int q) {
     switch (q) {
-        case 1: return slot_bytes<1, R>();
-        case 2: return slot_bytes<2, R>();
-        case 3: return slot_bytes<3, R>();
-        case 4: return slot_bytes<4, R>();
-        case 5: return slot_bytes<5, R>();
+        case 1: return SlotWork<1, R>::bytes;
+        case 2: return SlotWork<2, R>::bytes;
+        case 3: return SlotWork<3, R>::bytes;
+        case 4: return SlotWork<4, R>::bytes;
+        case 5: return SlotWork<5, R>::bytes;
         default: return 0;
     }
 }
 
+// Raise the dynamic-LDS limit of the Gauss-Jordan kernel to what it uses.  A
+// runtime that does not need the call may refuse it; the launch itself then
+// reports whether the size is accepted.
+template <int Q, typename R>
+static void allow_lds() {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&gj_round_kernel<Q, R>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)GjLay<Q, R>::bytes) != hipSuccess)
+        (void)hipGetLastError();
+}
+
 template <typename R>
-static int dispatch_launch(const DecimateArgs &a, int q, int slots, hipStream_t s) {
+static void dispatch_allow_lds(int q) {
     switch (q) {
-        case 1: return launch_q<1, R>(a, slots, s);
-        case 2: return launch_q<2, R>(a, slots, s);
-        case 3: return launch_q<3, R>(a, slots, s);
-        case 4: return launch_q<4, R>(a, slots, s);
-        case 5: return launch_q<5, R>(a, slots, s);
-        default: return -1;
+        case 1: allow_lds<1, R>(); break;
+        case 2: allow_lds<2, R>(); break;
+        case 3: allow_lds<3, R>(); break;
+        case 4: allow_lds<4, R>(); break;
+        case 5: allow_lds<5, R>(); break;
+        default: break;
     }
 }
 
-int decimate_slots(int q, int precision, int num_cu) {
-    const int per = precision == 0 ? dispatch_slots<double>(q) : dispatch_slots<float>(q);
-    return per > 0 ? per * num_cu : -1;
+int decimate_slots_per_set(int ninst, int num_cu) {
+    const int half = (ninst + 1) / 2;
+    return half < num_cu ? (half > 0 ? half : 1) : num_cu;
 }
 
 size_t decimate_slot_bytes(int q, int precision) {
     return precision == 0 ? dispatch_slot_bytes<double>(q) : dispatch_slot_bytes<float>(q);
 }
 
-int decimate_launch(const DecimateArgs &a, int q, int precision, int slots, hipStream_t s) {
-    return precision == 0 ? dispatch_launch<double>(a, q, slots, s)
-                          : dispatch_launch<float>(a, q, slots, s);
+static int launch_phase(const DecimateArgs &a, int q, int precision, int per_set, int set, bool gj,
+                        hipStream_t s) {
+    return precision == 0 ? dispatch_round<double>(a, q, per_set, set, gj, s)
+                          : dispatch_round<float>(a, q, per_set, set, gj, s);
+}
+
+int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const RoundCtl &ctl) {
+    hipStream_t s0 = ctl.s[0], s1 = ctl.s[1];
+    if (precision == 0)
+        dispatch_allow_lds<double>(q);
+    else
+        dispatch_allow_lds<float>(q);
+    // An instance takes at most itermax (inverse, GEMMs) rounds plus four
+    // inverse rounds (start, three finals); a slot runs ceil(ninst / slots)
+    // instances.  Past that bound (with slack) the device has stopped making
+    // progress and the run fails instead of launching rounds forever.
+    const long per_slot = (a.ninst + 2L * per_set - 1) / (2L * per_set) + 1;
+    const long max_batches = per_slot * (a.itermax + 4) / kRoundsPerBatch + 4;
+    if (hipMemsetAsync(a.state, 0, (size_t)2 * per_set * sizeof(SlotState), s0) != hipSuccess ||
+        hipMemsetAsync(a.queue, 0, 2 * sizeof(unsigned), s0) != hipSuccess ||
+        hipEventRecord(ctl.ev[2], s0) != hipSuccess || hipStreamWaitEvent(s1, ctl.ev[2], 0) != hipSuccess)
+        return -1;
+    // Stream 0 runs set 0 as (inverse, GEMMs) rounds, stream 1 runs set 1 as
+    // (GEMMs, inverse): the sets stay half a round apart.  Every batch of
+    // rounds ends with a copy of the finished-instance count; the count of
+    // the batch before is read while this one runs, so the device never
+    // waits for the host.
+    for (long b = 0;; ++b) {
+        if (b > max_batches) return -2;
+        for (int r = 0; r < kRoundsPerBatch; ++r) {
+            if (launch_phase(a, q, precision, per_set, 0, true, s0) ||
+                launch_phase(a, q, precision, per_set, 0, false, s0) ||
+                launch_phase(a, q, precision, per_set, 1, false, s1) ||
+                launch_phase(a, q, precision, per_set, 1, true, s1))
+                return -1;
+        }
+        if (hipMemcpyAsync(ctl.pinned + (b & 1), a.queue + 1, sizeof(unsigned),
+                           hipMemcpyDeviceToHost, s0) != hipSuccess ||
+            hipEventRecord(ctl.ev[b & 1], s0) != hipSuccess)
+            return -1;
+        if (b > 0) {
+            if (hipEventSynchronize(ctl.ev[(b - 1) & 1]) != hipSuccess) return -1;
+            if (ctl.pinned[(b - 1) & 1] >= (unsigned)a.ninst) break;
+        }
+    }
+    if (hipEventRecord(ctl.ev[2], s1) != hipSuccess || hipStreamWaitEvent(s0, ctl.ev[2], 0) != hipSuccess)
+        return -1;
+    return 0;
 }
 
 }  // namespace wtsurf

@@ -48,6 +48,11 @@ struct wtsurf_ctx {
     int num_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    // round driver of the decimation: a second stream, its events and a
+    // pinned progress counter
+    hipStream_t aux = nullptr;
+    hipEvent_t drv[3] = {nullptr, nullptr, nullptr};
+    unsigned *pinned = nullptr;
     std::string err;
 
     // model
@@ -69,9 +74,9 @@ struct wtsurf_ctx {
     DevBuf<double> dos_l, dos_r, dos_b;
     DevBuf<int> iters, status;
     // decimation workspace
-    DevBuf<unsigned char> work;
+    DevBuf<unsigned char> work, state;
     DevBuf<unsigned> queue;
-    int work_slots = 0;
+    int per_set = 0;
     bool ran = false;
 
     int fail(const std::string &m) {
@@ -120,17 +125,18 @@ static int sync_orbitals(wtsurf_ctx *c) {
     return 0;
 }
 
-static int ensure_work(wtsurf_ctx *c, int q, int precision) {
-    const int slots = decimate_slots(q, precision, c->num_cu);
-    if (slots <= 0) return c->fail("no decimation kernel for this order");
-    const size_t want = (size_t)slots * decimate_slot_bytes(q, precision);
+static int ensure_work(wtsurf_ctx *c, int q, int precision, int ninst) {
+    const size_t per_slot = decimate_slot_bytes(q, precision);
+    if (per_slot == 0) return c->fail("no decimation kernel for this order");
+    const int per_set = decimate_slots_per_set(ninst, c->num_cu);
+    const size_t want = (size_t)2 * per_set * per_slot;
     if (want > c->work.n) {
         c->work.release();
         if (c->work.ensure(want)) return c->fail("out of device memory (decimation workspace)");
-        WT_HIP(c, hipMemset(c->work.p, 0, want));
     }
-    if (c->queue.ensure(1)) return c->fail("out of device memory (queue)");
-    c->work_slots = slots;
+    if (c->state.ensure((size_t)2 * per_set * sizeof(SlotState)) || c->queue.ensure(2))
+        return c->fail("out of device memory (slot state)");
+    c->per_set = per_set;
     return 0;
 }
 
@@ -147,12 +153,19 @@ int wtsurf_ctx_create(int device, wtsurf_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return -2; }
     c->num_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&c->pinned, 2 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess) {
+        wtsurf_ctx_destroy(c);
         return -2;
     }
     for (auto &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) { delete c; return -2; }
+        if (hipEventCreate(&e) != hipSuccess) { wtsurf_ctx_destroy(c); return -2; }
+    for (auto &e : c->drv)
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            wtsurf_ctx_destroy(c);
+            return -2;
+        }
     *out = c;
     return 0;
 }
@@ -161,14 +174,20 @@ void wtsurf_ctx_destroy(wtsurf_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
     c->hr.release(); c->irvec.release(); c->slot_begin.release();
     c->d_top.release(); c->d_bot.release();
     c->kpts.release(); c->omega.release(); c->phase.release(); c->hk.release();
     c->dos_l.release(); c->dos_r.release(); c->dos_b.release();
-    c->iters.release(); c->status.release(); c->work.release(); c->queue.release();
+    c->iters.release(); c->status.release(); c->work.release(); c->state.release();
+    c->queue.release();
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto &e : c->drv)
+        if (e) (void)hipEventDestroy(e);
+    if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     delete c;
 }
 
@@ -298,21 +317,29 @@ static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const
                              int precision) {
     const int q = padded_q(n);
     if (q > decimate_max_q()) return c->fail("Ndim exceeds the largest built decimation kernel");
-    if (int r = ensure_work(c, q, precision)) return r;
+    if (int r = ensure_work(c, q, precision, ninst)) return r;
     if (c->dos_l.ensure(ninst) || c->dos_r.ensure(ninst) || c->dos_b.ensure(ninst) ||
         c->iters.ensure(ninst) || c->status.ensure(ninst))
         return c->fail("out of device memory (outputs)");
-    WT_HIP(c, hipMemsetAsync(c->queue.p, 0, sizeof(unsigned), c->stream));
     DecimateArgs a{};
     a.hk = c->hk.p; a.omega = d_omega; a.eta = nullptr; a.inst_k = nullptr; a.inst_w = nullptr;
     a.eta0 = eta; a.nk = nk; a.nomega = nomega; a.ninst = ninst; a.n = n; a.nw = nw;
     a.itermax = c->itermax; a.accuracy = c->accuracy;
     a.top = d_top; a.ntop = ntop; a.bot = d_bot; a.nbot = nbot; a.divisor = divisor;
-    a.work = c->work.p; a.queue = c->queue.p;
+    a.work = c->work.p; a.state = reinterpret_cast<SlotState *>(c->state.p); a.queue = c->queue.p;
     a.dos_l = c->dos_l.p; a.dos_r = c->dos_r.p; a.dos_bulk = c->dos_b.p;
     a.iters = c->iters.p; a.status = c->status.p; a.gfull = gfull;
-    const int slots = std::min(c->work_slots, ninst);
-    if (decimate_launch(a, q, precision, slots, c->stream)) return c->fail("decimation kernel launch failed");
+    RoundCtl ctl{};
+    ctl.s[0] = c->stream;
+    ctl.s[1] = c->aux;
+    for (int i = 0; i < 3; ++i) ctl.ev[i] = c->drv[i];
+    ctl.pinned = c->pinned;
+    if (int r = decimate_run(a, q, precision, c->per_set, ctl)) {
+        (void)hipStreamSynchronize(c->aux);  // rounds already queued on the second stream
+        if (r == -2) return c->fail("decimation made no progress within its round bound");
+        const hipError_t e = hipGetLastError();
+        return e != hipSuccess ? c->hipfail(e, "decimation rounds") : c->fail("decimation rounds failed");
+    }
     return 0;
 }
 
@@ -407,6 +434,7 @@ int wtsurf_sync(wtsurf_ctx *c) {
     if (!c) return -1;
     if (int r = use_device(c)) return r;
     WT_HIP(c, hipStreamSynchronize(c->stream));
+    WT_HIP(c, hipStreamSynchronize(c->aux));
     return 0;
 }
 

@@ -46,16 +46,26 @@ __device__ __forceinline__ float readlane_f32(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// Maximum over the 64 lanes of a wave, returned to every lane: DPP row
-// shifts inside each row of 16 lanes, then row broadcasts 15 and 31 carry
-// the partial maxima up to lane 63.
+// max(a, b) (a quiet NaN operand yields the other one, as fmax) for operands
+// that are never signalling NaNs, which no arithmetic produces: one
+// v_max_f64, without the quieting step fmax adds on values the compiler
+// cannot prove canonical (every DPP-moved value).
+__device__ __forceinline__ double fmax_nocanon(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
+// Maximum over the 64 lanes of a wave (operands as fmax_nocanon), returned
+// to every lane: DPP row shifts inside each row of 16 lanes, then
+// row broadcasts 15 and 31 carry the partial maxima up to lane 63.
 __device__ __forceinline__ double wave_max_f64(double v) {
 #define WT_DPP_MAX_STEP(ctrl, rmask)                                                  \
     {                                                                                  \
         const int hi = __double2hiint(v), lo = __double2loint(v);                      \
         const int h2 = __builtin_amdgcn_update_dpp(hi, hi, ctrl, rmask, 0xf, false);   \
         const int l2 = __builtin_amdgcn_update_dpp(lo, lo, ctrl, rmask, 0xf, false);   \
-        v = fmax(v, __hiloint2double(h2, l2));                                         \
+        v = fmax_nocanon(v, __hiloint2double(h2, l2));                                 \
     }
     WT_DPP_MAX_STEP(0x111, 0xf)  // row_shr:1
     WT_DPP_MAX_STEP(0x112, 0xf)  // row_shr:2
@@ -82,41 +92,28 @@ __device__ __forceinline__ C cadd(C a, C b) {
     r.y = a.y + b.y;
     return r;
 }
-// 1/z by Smith's scaling (one real division; no overflow for large |z|).
-template <typename C>
-__device__ __forceinline__ C crecip(C z) {
-    C out;
-    if (fabs(z.x) >= fabs(z.y)) {
-        const auto r = z.y / z.x;
-        const auto inv = decltype(r)(1) / (z.x + z.y * r);
-        out.x = inv;
-        out.y = -r * inv;
-    } else {
-        const auto r = z.x / z.y;
-        const auto inv = decltype(r)(1) / (z.y + z.x * r);
-        out.x = r * inv;
-        out.y = -inv;
-    }
-    return out;
+// 1/z = conj(z) / |z|^2 without a division: both parts are scaled by 2^-e,
+// e the binary exponent of max(|re|, |im|), so that |z|^2 lies in [1/4, 2)
+// (no overflow or underflow whatever the magnitude of z); the hardware
+// reciprocal of |z|^2 is refined by Newton steps (two in FP64, one in FP32:
+// each step doubles the correct bits of the ~2^-23 estimate) and the result
+// is scaled back by 2^-e.  A few ulp from the correctly rounded quotient.
+__device__ __forceinline__ double2 crecip_fast(double2 z) {
+    const int e = __builtin_amdgcn_frexp_exp(fmax(fabs(z.x), fabs(z.y)));
+    const double x = __builtin_amdgcn_ldexp(z.x, -e), y = __builtin_amdgcn_ldexp(z.y, -e);
+    const double d = fma(x, x, y * y);
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return make_double2(__builtin_amdgcn_ldexp(x * r, -e), __builtin_amdgcn_ldexp(-y * r, -e));
 }
-
-// crecip for a value every lane holds identically: the scaling branch is
-// taken as one uniform branch instead of evaluating both sides.
-template <typename C>
-__device__ __forceinline__ C crecip_uniform(C z) {
-    C out;
-    if (uniform(fabs(z.x) >= fabs(z.y) ? 1 : 0)) {
-        const auto r = z.y / z.x;
-        const auto inv = decltype(r)(1) / (z.x + z.y * r);
-        out.x = inv;
-        out.y = -r * inv;
-    } else {
-        const auto r = z.x / z.y;
-        const auto inv = decltype(r)(1) / (z.y + z.x * r);
-        out.x = r * inv;
-        out.y = -inv;
-    }
-    return out;
+__device__ __forceinline__ float2 crecip_fast(float2 z) {
+    const int e = __builtin_amdgcn_frexp_expf(fmaxf(fabsf(z.x), fabsf(z.y)));
+    const float x = __builtin_amdgcn_ldexpf(z.x, -e), y = __builtin_amdgcn_ldexpf(z.y, -e);
+    const float d = fmaf(x, x, y * y);
+    float r = __builtin_amdgcn_rcpf(d);
+    r = fmaf(r, fmaf(-d, r, 1.0f), r);
+    return make_float2(__builtin_amdgcn_ldexpf(x * r, -e), __builtin_amdgcn_ldexpf(-y * r, -e));
 }
 
 // Per-precision MFMA: one 16x16x4 step, D += A B, with the operand layout

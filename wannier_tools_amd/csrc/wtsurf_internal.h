@@ -7,6 +7,15 @@
 
 namespace wtsurf {
 
+// Decimation state of one instance slot between rounds (device memory).
+struct SlotState {
+    int phase;              // empty, inverse next, GEMMs next, final inverse 0..2, done
+    int t;                  // instance id
+    int it;                 // current iteration (1-based)
+    int status;             // bit0 singular solve, bit1 not converged
+    double dl, dr;          // traces of the final inverses done so far
+};
+
 // Arguments of one batched decimation launch (surfgreen_1985 over many
 // (k, omega) instances, src/surfgreen.f90:7-137, with the trace epilogue of
 // src/surfstat.f90:150-160 / src/fermiarc.f90:238-248).
@@ -26,20 +35,33 @@ struct DecimateArgs {
     const int *bot;
     int nbot;
     double divisor;         // 1 (surfstat) or pi (SurfaceDOSkk)
-    void *work;             // [slots][decimate_slot_bytes], zeroed once at allocation
-    unsigned *queue;        // instance counter, zero at launch
+    void *work;             // [2*per_set][decimate_slot_bytes]
+    SlotState *state;       // [2*per_set], zeroed by decimate_run
+    unsigned *queue;        // [2]: next instance ticket, finished instances
     double *dos_l, *dos_r, *dos_bulk;  // [ninst] in (ik + nk*iw) order
     int *iters;             // [ninst]
     int *status;            // [ninst] bit0 singular solve, bit1 not converged
     double2 *gfull;         // [ninst][3][n*n] GLL, GRR, GB or null
 };
 
+// Streams and host-visible progress counter of the round driver: two
+// streams at default priority, three events, two pinned unsigneds.
+struct RoundCtl {
+    hipStream_t s[2];
+    hipEvent_t ev[3];
+    unsigned *pinned;
+};
+
 int padded_q(int n);        // NP/16 for order n
 int decimate_max_q();       // largest Q the decimation kernels are built for
+// Slots per stream set for ninst instances on num_cu CUs (one of each kernel per CU).
+int decimate_slots_per_set(int ninst, int num_cu);
 // precision: 0 = FP64, 1 = FP32 inverse/GEMMs with FP64 accumulation and finals
-int decimate_slots(int q, int precision, int num_cu);
 size_t decimate_slot_bytes(int q, int precision);
-int decimate_launch(const DecimateArgs &a, int q, int precision, int slots, hipStream_t s);
+// Enqueues the decimation of a.ninst instances on ctl.s[0] and ctl.s[1] and
+// returns once every instance is counted finished (the work is complete when
+// ctl.s[0] reaches the end of what was enqueued; ctl.s[0] waits for s[1]).
+int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const RoundCtl &ctl);
 
 // Fourier sum H00(k), H01(k) (src/ham_qlayer2qlayer.f90:45-97).
 struct FourierArgs {
