@@ -9,8 +9,15 @@ k points, energies) are resident on the device before timing starts; a step
 is the Fourier sum plus the batched decimation and traces.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32_refine|auto]
+    python bench.py --summarize-rocprof TRACE_DIR FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
 
-Prints one JSON line.  The FP64 path always runs (it defines the per-instance
+Prints one JSON line.  The decimation runs as rounds of two kernels on two
+streams (gj_round_kernel and gemm_round_kernel); the roofline figures refer to
+that whole decimation stage, timed with HIP events on the first stream, which
+waits for the second at the end of the stage.  --summarize-rocprof turns the
+rocprofv3 runs of the same command (kernel trace; FETCH_SIZE and WRITE_SIZE
+counters; the same counters on a plain 1 GiB copy for calibration) into the
+summary kept under profiles/.  The FP64 path always runs (it defines the per-instance
 reference iteration count I_ref used for the algorithmic flop count
 F = (56 I_ref + 24) N^3 per solve).  The reported `value` is the solves/s of
 the fastest path whose A(k, omega) error against FP64 is <= 1e-6.
@@ -32,6 +39,8 @@ if ROOT not in sys.path:
 
 from wannier_tools_amd import model as M  # noqa: E402
 
+TRAFFIC_JSON = "decimation_fp64_ndim72_512x512.traffic.json"
+DECIMATION_KERNELS = ("gj_round_kernel", "gemm_round_kernel")
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
@@ -124,7 +133,57 @@ def cpu_baseline(kpts, omegas, eta, irvec, hr, per_worker: int, workers: int):
     }
 
 
+def _csv_rows(d: str, suffix: str):
+    import csv
+    import glob
+    paths = glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True)
+    if not paths:
+        raise SystemExit(f"no *{suffix} under {d}")
+    with open(paths[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def _counter_kib(d: str, counter: str, kernels) -> float:
+    """Sum of a rocprofv3 counter over the dispatches of the named kernels (KiB)."""
+    return sum(float(r["Counter_Value"]) for r in _csv_rows(d, "counter_collection.csv")
+               if r["Counter_Name"] == counter and any(k in r["Kernel_Name"] for k in kernels))
+
+
+def summarize_rocprof(trace, fetch, write, cfetch, cwrite) -> int:
+    """profiles/ summary of the decimation stage from rocprofv3 runs of
+    `bench.py --steps 1 --warmup 0 --no-cpu-baseline --precision fp64`."""
+    stats = _csv_rows(trace, "kernel_stats.csv")
+    rows = _csv_rows(trace, "kernel_trace.csv")
+    dec = [r for r in rows if any(k in r["Kernel_Name"] for k in DECIMATION_KERNELS)]
+    span_ns = max(int(r["End_Timestamp"]) for r in dec) - min(int(r["Start_Timestamp"]) for r in dec)
+    busy = {k: sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in dec
+                   if k in r["Kernel_Name"]) for k in DECIMATION_KERNELS}
+    # counter scale: KiB reported for a copy of known size
+    copy_bytes = 1 << 30
+    f_scale = copy_bytes / (_counter_kib(cfetch, "FETCH_SIZE", ("copyBuffer",)) * 1024)
+    w_scale = copy_bytes / (_counter_kib(cwrite, "WRITE_SIZE", ("copyBuffer",)) * 1024)
+    fetch_b = _counter_kib(fetch, "FETCH_SIZE", DECIMATION_KERNELS) * 1024 * f_scale
+    write_b = _counter_kib(write, "WRITE_SIZE", DECIMATION_KERNELS) * 1024 * w_scale
+    json.dump({"stage": "decimation rounds (gj_round_kernel + gemm_round_kernel)",
+               "workload": "ndim72 512x512 eta=1e-3 eV", "fetch_bytes": fetch_b,
+               "write_bytes": write_b, "fetch_scale": f_scale, "write_scale": w_scale,
+               "span_ns": span_ns}, open(os.path.join(ROOT, "profiles", TRAFFIC_JSON), "w"),
+              indent=1)
+    print("| Kernel | Calls | Total ms | Average us |\n|---|---|---|---|")
+    for r in stats[:8]:
+        print(f"| `{r['Name'][:90]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
+              f"{float(r['AverageNs']) / 1e3:.1f} |")
+    print(f"\nDecimation stage span (first round start to last round end): {span_ns / 1e6:.1f} ms; "
+          + "; ".join(f"{k} busy {v / 1e6:.1f} ms" for k, v in busy.items()))
+    print(f"\nHBM traffic of the stage: fetch {fetch_b:.4e} B (scale {f_scale:.3f}), "
+          f"write {write_b:.4e} B (scale {w_scale:.3f}), {(fetch_b + write_b) / span_ns:.1f} GB/s "
+          f"over the span")
+    return 0
+
+
 def main() -> int:
+    if len(sys.argv) == 7 and sys.argv[1] == "--summarize-rocprof":
+        return summarize_rocprof(*sys.argv[2:])
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
@@ -204,7 +263,7 @@ def main() -> int:
     # HBM bytes per launch of this kernel on this workload, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a 1 GiB copy calibration)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "decimate_fp64_ndim72_512x512.traffic.json")
+    tpath = os.path.join(ROOT, "profiles", TRAFFIC_JSON)
     if name == "fp64" and (args.nk, args.nomega) == (512, 512) and os.path.exists(tpath):
         t = json.load(open(tpath))
         traffic = t["fetch_bytes"] + t["write_bytes"]
@@ -228,7 +287,9 @@ def main() -> int:
                    "path": name},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": traffic,
-                     "kernel": "decimate_kernel<5, double>" if name == "fp64" else "decimate_kernel<5, float>",
+                     "kernel": ("decimation stage: gj_round_kernel<5, %s> + gemm_round_kernel<5, %s> "
+                                "rounds on two streams" % ((("double",) * 2) if name == "fp64" else
+                                                           (("float",) * 2))),
                      "flops_per_launch": flops},
         "paths": res,
     }
