@@ -123,6 +123,60 @@ def test_uneven_orders_vs_oracle(gpu, oracle_lib, nw, np_):
     _traces_within_floor((dl, dr, db), (odl, odr, odb), floor)
 
 
+@pytest.mark.parametrize("nw,np_", [(48, 2), (50, 2), (36, 3), (72, 3), (144, 2)])
+def test_large_orders_vs_oracle(gpu, oracle_lib, nw, np_):
+    """Orders beyond 80, where the Gauss-Jordan matrix lives in global memory
+    and the GEMMs run in blocks: Ndim 96 (Q 6), 100 (padded to Q 8), 108, 216
+    (the largest order of the mixed-model set) and 288 (the 144-orbital spinful
+    k-plane model), with custom orbital lists and the trace divisor pi."""
+    _say("large orders", nw, np_)
+    mdl = M.synthetic_model(nw, 2, 0.5, M.decay_l2(1.5), seed=200 + nw)
+    irvec, hr = mdl.surface_cell()
+    gpu.set_model_arrays(nw, np_, irvec, hr)
+    n = nw * np_
+    top = np.array(sorted({1, 2, n // 3, n}), dtype=np.int32)
+    bot = np.array(sorted({1, nw // 2, nw}), dtype=np.int32)
+    gpu.set_orbitals(top, bot)
+    kp = np.array([[0.13, -0.21], [-0.4, 0.05]])
+    om = np.array([-0.3, 0.05]) * M.EV2HARTREE
+    eta = 2e-2 * M.EV2HARTREE
+    dl, dr, db, its, st = gpu.spectral_grid(kp, om, eta, divisor=M.REF_PI)
+    odl, odr, odb, oits = oracle_lib.spectral(nw, np_, irvec, hr, kp, om, eta, top, bot,
+                                              divisor=M.REF_PI)
+    assert np.all(st == 0)
+    assert np.array_equal(its, oits)
+    floor = fp64_floor(oracle_lib, nw, np_, irvec, hr, kp, om, eta, top, bot, divisor=M.REF_PI,
+                       draws=2)
+    _traces_within_floor((dl, dr, db), (odl, odr, odb), floor)
+
+
+def test_large_order_surfgreen_full_g(gpu, oracle_lib):
+    """surfgreen_1985 at Ndim 216: the full GLL, GRR, GB from the global-memory
+    inverse against the oracle."""
+    _say("large surfgreen")
+    mdl = M.synthetic_model(72, 2, 0.5, M.decay_l2(1.5), seed=272)
+    irvec, hr = mdl.surface_cell()
+    gpu.set_model_arrays(72, 3, irvec, hr)
+    h00, h01 = gpu.ham_qlayer2qlayer(np.array([[0.2, 0.1]]))
+    w, eta = 0.1 * M.EV2HARTREE, 2e-2 * M.EV2HARTREE
+    gll, grr, gb, it = gpu.surfgreen_1985(w, h00[0], h01[0], eta)
+    ogll, ogrr, ogb, oit, info = oracle_lib.surfgreen_1985(w, eta, h00[0], h01[0])
+    assert info == 0 and it == oit
+    for mine, ref in zip((gll, grr, gb), (ogll, ogrr, ogb)):
+        assert np.max(np.abs(mine - ref)) < 1e-10 * np.max(np.abs(ref))
+
+
+def test_order_beyond_largest_kernel_fails(gpu):
+    from wannier_tools_amd import surface
+    nmax = surface.max_ndim()
+    assert nmax >= 288
+    mdl = M.synthetic_model(nmax // 2 + 1, 1, 0.3, M.decay_l1, seed=5)
+    irvec, hr = mdl.surface_cell()
+    gpu.set_model_arrays(nmax // 2 + 1, 2, irvec, hr)
+    with pytest.raises(WtsurfError):
+        gpu.spectral_grid([[0.0, 0.0]], [0.0], 1e-3)
+
+
 def test_chain_closed_form_on_device(gpu):
     _say("chain")
     e0, t = 0.2 * M.EV2HARTREE, -0.8 * M.EV2HARTREE

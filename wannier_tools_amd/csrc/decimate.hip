@@ -89,109 +89,100 @@ struct Common {
 // X^-1[i][j] = S[cm.pivstep[j]][cm.pivrow[i]].
 
 // Panel elimination by one wave: columns c0 .. c0+kPanel-1 of S, rows
-// r0 = lane and r1 = lane + 64, held in registers for the whole panel (no
-// barrier per step; the pivot search is a DPP wave maximum plus a ballot).
-// Records the pivots in cm.pivrow / cm.pivstep and writes the panel back.
-// Returns true (every lane) on an exactly zero pivot column.
+// lane + 64 s (s < NSET, the row sets), held in registers for the whole
+// panel (no barrier per step; the pivot search is a DPP wave maximum plus a
+// ballot).  Records the pivots in cm.pivrow / cm.pivstep and writes the panel
+// back.  Returns true (every lane) on an exactly zero pivot column.
 template <int Q, typename R>
 __device__ __forceinline__ bool gj_panel(typename Prec<R>::C (*S)[16 * Q + 1], Common<Q> &cm,
                                          int c0, int n, int lane) {
     using P = Prec<R>;
     using C = typename P::C;
     constexpr int NP = 16 * Q;
-    const int r0 = lane < NP ? lane : 0, r1 = lane + 64;
-    const bool has0 = lane < n;
-    const bool two = NP > 64 && uniform(n > 64 ? 1 : 0);  // second rows exist
-    const bool has1 = two && r1 < n;
-    C a0[kPanel], a1[kPanel];
+    constexpr int NSET = (NP + 63) / 64;
+    C a[NSET][kPanel];
+    bool has[NSET], fr[NSET];
+    int live[NSET];  // row set s holds rows of the matrix at all (uniform)
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
-        a0[j] = S[c0 + j][r0];
-        if (!has0) a0[j].x = a0[j].y = 0;
-        a1[j].x = a1[j].y = 0;
-        if (has1) a1[j] = S[c0 + j][r1];
+    for (int s = 0; s < NSET; ++s) {
+        const int r = lane + 64 * s;
+        live[s] = s == 0 ? 1 : uniform(64 * s < n ? 1 : 0);
+        has[s] = r < n;
+#pragma unroll
+        for (int j = 0; j < kPanel; ++j) {
+            a[s][j].x = a[s][j].y = 0;
+            if (has[s]) a[s][j] = S[c0 + j][r];
+        }
+        fr[s] = has[s] && cm.pivstep[has[s] ? r : 0] < 0;
     }
-    bool free0 = has0 && cm.pivstep[r0] < 0;
-    bool free1 = has1 && cm.pivstep[has1 ? r1 : 0] < 0;
     bool sing = false;
 #pragma unroll
     for (int j = 0; j < kPanel; ++j) {
         const int c = c0 + j;
         if (c >= n) break;
-        const double v0 = free0 ? (double)(fabs(a0[j].x) + fabs(a0[j].y)) : -1.0;
-        const double v1 = free1 ? (double)(fabs(a1[j].x) + fabs(a1[j].y)) : -1.0;
-        const double best = wave_max_f64(two ? fmax_nocanon(v0, v1) : v0);
+        double v[NSET];
+        double vmax = -1.0;
+#pragma unroll
+        for (int s = 0; s < NSET; ++s) {
+            v[s] = fr[s] ? (double)(fabs(a[s][j].x) + fabs(a[s][j].y)) : -1.0;
+            vmax = s == 0 ? v[0] : fmax_nocanon(vmax, v[s]);
+        }
+        const double best = wave_max_f64(vmax);
         if (!(best > 0.0)) {  // zero (or NaN) pivot column: singular
             sing = true;
             break;
         }
-        // lowest row attaining the maximum: rows 0..63 before 64..127
-        const unsigned long long m0 = __ballot(v0 == best);
-        const unsigned long long m1 = two ? __ballot(v1 == best) : 0ull;
-        const int slot = uniform(m0 ? 0 : 1);
-        const int pl = uniform(__ffsll((long long)(m0 ? m0 : m1)) - 1);
-        const int p = pl + 64 * slot;
-        // pivot row from lane pl
-        C row[kPanel];
-        if (slot == 0) {
+        // lowest row attaining the maximum: set 0 before set 1 before ...
+        int slot = NSET, pl = 0;
 #pragma unroll
-            for (int jj = 0; jj < kPanel; ++jj) {
-                row[jj].x = P::readlane(a0[jj].x, pl);
-                row[jj].y = P::readlane(a0[jj].y, pl);
-            }
-        } else {
-#pragma unroll
-            for (int jj = 0; jj < kPanel; ++jj) {
-                row[jj].x = P::readlane(a1[jj].x, pl);
-                row[jj].y = P::readlane(a1[jj].y, pl);
+        for (int s = NSET - 1; s >= 0; --s) {
+            const unsigned long long m = __ballot(v[s] == best);
+            if (m) {
+                slot = s;
+                pl = __ffsll((long long)m) - 1;
             }
         }
+        slot = uniform(slot);
+        pl = uniform(pl);
+        const int p = pl + 64 * slot;
+        // pivot row from lane pl of row set `slot`
+        C row[kPanel];
+#pragma unroll
+        for (int s = 0; s < NSET; ++s)
+            if (slot == s) {
+#pragma unroll
+                for (int jj = 0; jj < kPanel; ++jj) {
+                    row[jj].x = P::readlane(a[s][jj].x, pl);
+                    row[jj].y = P::readlane(a[s][jj].y, pl);
+                }
+            }
         const C r = crecip_fast(row[j]);
         // eliminate column c from every row with the multiplier m = f r
         // (f = a[j], r = 1/pivot):  a[jj] = s a[jj] - m row[jj] for jj != j,
         // a[j] = -m, s = 1.  The pivot row takes m = -r, s = 0, which leaves
         // r row[jj] (the normalised pivot row) in it and r in entry j.
-        {
-            const bool isp = slot == 0 && lane == pl;
-            C m = cmul(a0[j], r);
-            R s = 1;
+#pragma unroll
+        for (int s = 0; s < NSET; ++s) {
+            if (!live[s]) continue;
+            const bool isp = slot == s && lane == pl;
+            C m = cmul(a[s][j], r);
+            R sc = 1;
             if (isp) {
                 m.x = -r.x;
                 m.y = -r.y;
-                s = 0;
+                sc = 0;
             }
 #pragma unroll
             for (int jj = 0; jj < kPanel; ++jj) {
                 if (jj == j) {
-                    a0[jj].x = -m.x;
-                    a0[jj].y = -m.y;
+                    a[s][jj].x = -m.x;
+                    a[s][jj].y = -m.y;
                 } else {
-                    a0[jj].x = s * a0[jj].x - (m.x * row[jj].x - m.y * row[jj].y);
-                    a0[jj].y = s * a0[jj].y - (m.x * row[jj].y + m.y * row[jj].x);
+                    a[s][jj].x = sc * a[s][jj].x - (m.x * row[jj].x - m.y * row[jj].y);
+                    a[s][jj].y = sc * a[s][jj].y - (m.x * row[jj].y + m.y * row[jj].x);
                 }
             }
-            free0 = free0 && !isp;
-        }
-        if (two) {
-            const bool isp = slot == 1 && lane == pl;
-            C m = cmul(a1[j], r);
-            R s = 1;
-            if (isp) {
-                m.x = -r.x;
-                m.y = -r.y;
-                s = 0;
-            }
-#pragma unroll
-            for (int jj = 0; jj < kPanel; ++jj) {
-                if (jj == j) {
-                    a1[jj].x = -m.x;
-                    a1[jj].y = -m.y;
-                } else {
-                    a1[jj].x = s * a1[jj].x - (m.x * row[jj].x - m.y * row[jj].y);
-                    a1[jj].y = s * a1[jj].y - (m.x * row[jj].y + m.y * row[jj].x);
-                }
-            }
-            free1 = free1 && !isp;
+            fr[s] = fr[s] && !isp;
         }
         if (lane == 0) {
             cm.pivrow[c] = p;
@@ -199,10 +190,10 @@ __device__ __forceinline__ bool gj_panel(typename Prec<R>::C (*S)[16 * Q + 1], C
         }
     }
 #pragma unroll
-    for (int j = 0; j < kPanel; ++j) {
-        if (lane < NP) S[c0 + j][r0] = a0[j];
-        if (has1) S[c0 + j][r1] = a1[j];
-    }
+    for (int s = 0; s < NSET; ++s)
+        if (has[s])
+#pragma unroll
+            for (int j = 0; j < kPanel; ++j) S[c0 + j][lane + 64 * s] = a[s][j];
     return sing;
 }
 
@@ -261,7 +252,9 @@ __device__ __forceinline__ void gj_update_tile(typename Prec<R>::C (*S)[16 * Q +
 }
 
 // Inverse of X = z*1 - E (the first n x n block of the padded FP64 matrix E,
-// or of E + E2 - H when E2 is given) in precision R, by NT threads.
+// or of E + E2 - H when E2 is given) in precision R, by NT threads.  X is
+// held in sbuf (NP x (NP+1), LDS for orders up to 80, the slot's global
+// workspace beyond), the panel pivot rows T in tbuf (LDS).
 //
 // Schedule with one panel of lookahead: after panel p is eliminated and its
 // pivot rows gathered into T, all waves apply p's update to the tile column
@@ -278,7 +271,7 @@ __device__ __forceinline__ void gj_update_tile(typename Prec<R>::C (*S)[16 * Q +
 // zero pivot and inv copies back the identity right-hand side,
 // src/inverse.f90:44-55).
 template <int Q, typename R, int NT>
-__device__ __forceinline__ bool gj_inverse(unsigned char *lds, Common<Q> &cm,
+__device__ __forceinline__ bool gj_inverse(unsigned char *sbuf, unsigned char *tbuf, Common<Q> &cm,
                                            const double2 *__restrict__ E,
                                            const double2 *__restrict__ E2,
                                            const double2 *__restrict__ H, double2 z, int n,
@@ -289,13 +282,13 @@ __device__ __forceinline__ bool gj_inverse(unsigned char *lds, Common<Q> &cm,
     constexpr int NP = 16 * Q;
     constexpr int LD = NP + 1;
     constexpr int NWAVES = NT / kWave;
-    static_assert(NP <= 128, "the panel wave holds at most two rows per lane");
+    static_assert(NP <= 320, "the panel wave holds at most five rows per lane");
     static_assert(NWAVES >= 2, "the lookahead needs a panel wave and update waves");
     const int tid = threadIdx.x;
     const int wave = uniform(tid >> 6), lane = tid & 63;
     const int lr = lane & 15, lq = lane >> 4;
-    C(*S)[LD] = reinterpret_cast<C(*)[LD]>(lds);
-    C(*T)[NP] = reinterpret_cast<C(*)[NP]>(lds + (size_t)NP * LD * sizeof(C));
+    C(*S)[LD] = reinterpret_cast<C(*)[LD]>(sbuf);
+    C(*T)[NP] = reinterpret_cast<C(*)[NP]>(tbuf);
 
     for (int e = tid; e < NP * NP; e += NT) {
         const int i = e % NP, j = e / NP;
@@ -638,6 +631,150 @@ __device__ __forceinline__ double dual_gemm(unsigned char *lds, double *red,
     return 0.0;
 }
 
+// LDS of the blocked GEMM (orders beyond 80): one K-chunk of the A0 and A1
+// row blocks (16 BI rows) and of the B column block (16 BJ columns, padded).
+constexpr int kBI = 6, kBJ = 4;  // output block of the blocked GEMM, in tiles
+template <int Q, typename R>
+struct GemmLayL {
+    using C = typename Prec<R>::C;
+    static constexpr size_t A = (size_t)kChunk * 16 * kBI * sizeof(C);
+    static constexpr size_t B = (size_t)kChunk * (16 * kBJ + 1) * sizeof(C);
+    static constexpr size_t bytes = 2 * A + B;
+};
+
+// The dual GEMM of dual_gemm (same products and MODEs) for orders whose
+// Q x Q tile pairs do not fit the waves' registers: the output is covered in
+// blocks of kBI x kBJ tile pairs, one pass over K per block, the block's A0 /
+// A1 rows and B columns staged through LDS by K-chunks.  kBI kBJ = 24 pairs
+// per block, three per wave.  B is read in every pass, so O1 must not be B.
+template <int Q, typename R, int MODE>
+__device__ __forceinline__ double dual_gemm_blocked(unsigned char *lds, double *red,
+                                                    const typename Prec<R>::C *__restrict__ A0,
+                                                    const typename Prec<R>::C *__restrict__ A1,
+                                                    const typename Prec<R>::C *__restrict__ B,
+                                                    int kdim, typename Prec<R>::C *__restrict__ O0,
+                                                    typename Prec<R>::C *__restrict__ O1,
+                                                    double2 *__restrict__ E) {
+    using P = Prec<R>;
+    using C = typename P::C;
+    using V4 = typename P::V4;
+    constexpr int NP = 16 * Q;
+    constexpr int RI = 16 * kBI, CJ = 16 * kBJ;
+    constexpr int NBI = (Q + kBI - 1) / kBI, NBJ = (Q + kBJ - 1) / kBJ;
+    constexpr int PPW = kBI * kBJ / kWaves;
+    static_assert(PPW * kWaves == kBI * kBJ, "block pairs must split evenly over the waves");
+    static_assert(kChunk * CJ == kBlock, "one B element per thread and chunk");
+    static_assert(kChunk * RI <= 2 * kBlock, "at most two A elements per thread and chunk");
+    const int tid = threadIdx.x;
+    const int wave = uniform(tid >> 6), lane = tid & 63;
+    const int lr = lane & 15, lq = lane >> 4;
+    C(*as0)[RI] = reinterpret_cast<C(*)[RI]>(lds);
+    C(*as1)[RI] = reinterpret_cast<C(*)[RI]>(lds + GemmLayL<Q, R>::A);
+    C(*bs)[CJ + 1] = reinterpret_cast<C(*)[CJ + 1]>(lds + 2 * GemmLayL<Q, R>::A);
+    const int nchunks = (kdim + kChunk - 1) / kChunk;
+    // staging: A elements e = tid, tid + kBlock of a chunk (kk = e / RI, row e % RI),
+    // B element tid (kk = tid % kChunk, column tid / kChunk)
+    const int ea1 = tid + kBlock;
+    const bool ha1 = ea1 < kChunk * RI;
+    const int ka0 = tid / RI, ra0 = tid % RI, ka1 = ha1 ? ea1 / RI : 0, ra1 = ea1 % RI;
+    const int kb = tid % kChunk, cb = tid / kChunk;
+
+    double asum = 0.0;
+    for (int blk = 0; blk < NBI * NBJ; ++blk) {
+        const int i0 = kBI * (blk % NBI), j0 = kBJ * (blk / NBI);
+        const int ni = Q - i0 < kBI ? Q - i0 : kBI, nj = Q - j0 < kBJ ? Q - j0 : kBJ;
+        int ti[PPW], tj[PPW];
+        bool act[PPW];
+        V4 pR[PPW][2], pI[PPW][2];
+#pragma unroll
+        for (int s = 0; s < PPW; ++s) {
+            const int pp = wave + kWaves * s;
+            act[s] = pp < ni * nj;
+            ti[s] = act[s] ? pp % ni : 0;
+            tj[s] = act[s] ? pp / ni : 0;
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                pR[s][w] = V4{0, 0, 0, 0};
+                pI[s][w] = V4{0, 0, 0, 0};
+            }
+        }
+        const bool va0 = 16 * i0 + ra0 < NP, va1 = ha1 && 16 * i0 + ra1 < NP;
+        const bool vb = 16 * j0 + cb < NP;
+        C x0, y0, x1, y1, b0;
+        auto load = [&](int c) {
+            const int k0 = c * kChunk;
+            x0.x = x0.y = y0.x = y0.y = x1.x = x1.y = y1.x = y1.y = b0.x = b0.y = 0;
+            if (va0) {
+                x0 = A0[16 * i0 + ra0 + (k0 + ka0) * NP];
+                y0 = A1[16 * i0 + ra0 + (k0 + ka0) * NP];
+            }
+            if (va1) {
+                x1 = A0[16 * i0 + ra1 + (k0 + ka1) * NP];
+                y1 = A1[16 * i0 + ra1 + (k0 + ka1) * NP];
+            }
+            if (vb) b0 = B[k0 + kb + (16 * j0 + cb) * NP];
+        };
+        auto stash = [&]() {
+            as0[ka0][ra0] = x0;
+            as1[ka0][ra0] = y0;
+            if (ha1) {
+                as0[ka1][ra1] = x1;
+                as1[ka1][ra1] = y1;
+            }
+            bs[kb][cb] = b0;
+        };
+        load(0);
+        for (int c = 0; c < nchunks; ++c) {
+            stash();
+            __syncthreads();
+            if (c + 1 < nchunks) load(c + 1);
+#pragma unroll
+            for (int ks = 0; ks < kChunk / 4; ++ks) {
+                const int kk = 4 * ks + lq;
+#pragma unroll
+                for (int s = 0; s < PPW; ++s) {
+                    if (!act[s]) continue;
+                    const C b = bs[kk][16 * tj[s] + lr];
+                    const C x = as0[kk][16 * ti[s] + lr];
+                    const C y = as1[kk][16 * ti[s] + lr];
+                    const R nby = -b.y;
+                    cmfma<R>(pR[s][0], pI[s][0], x, b, nby);
+                    cmfma<R>(pR[s][1], pI[s][1], y, b, nby);
+                }
+            }
+            __syncthreads();
+        }
+        if (MODE == 0) {
+#pragma unroll
+            for (int s = 0; s < PPW; ++s)
+                if (act[s]) store_tile<Q, R, false>(i0 + ti[s], j0 + tj[s], lr, lq, pR[s][0], pI[s][0], O0, asum);
+        } else {
+            ETile t[PPW];
+#pragma unroll
+            for (int s = 0; s < PPW; ++s)
+                if (act[s]) load_etile<Q, R>(t[s], i0 + ti[s], j0 + tj[s], lr, lq, E);
+#pragma unroll
+            for (int s = 0; s < PPW; ++s)
+                if (act[s]) add_etile<Q, R>(t[s], i0 + ti[s], j0 + tj[s], lr, lq, pR[s][0], pI[s][0], E);
+        }
+#pragma unroll
+        for (int s = 0; s < PPW; ++s)
+            if (act[s])
+                store_tile<Q, R, MODE == 2>(i0 + ti[s], j0 + tj[s], lr, lq, pR[s][1], pI[s][1], O1, asum);
+    }
+    if (MODE == 2) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) asum += __shfl_xor(asum, off, 64);
+        if (lane == 0) red[wave] = asum;
+        __syncthreads();
+        double tot = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) tot += red[w];
+        return tot;
+    }
+    return 0.0;
+}
+
 // Per-slot workspace: alpha, beta, m1, m2, g in precision R, then eps_s and
 // eps_t in FP64, each NP x NP column-major.
 template <int Q, typename R>
@@ -647,6 +784,12 @@ struct SlotWork {
     static constexpr size_t bytes = 5 * NN * sizeof(C) + 2 * NN * sizeof(double2);
     C *al, *be, *m1, *m2, *g;
     double2 *es, *et;
+    // alpha / beta of the current iteration and where the next ones go: the
+    // single-pass GEMMs overwrite them in place
+    __device__ C *alpha(int) const { return al; }
+    __device__ C *beta(int) const { return be; }
+    __device__ C *alpha_next(int) const { return al; }
+    __device__ C *beta_next(int) const { return be; }
     __device__ SlotWork(void *work, int slot) {
         unsigned char *base = reinterpret_cast<unsigned char *>(work) + (size_t)slot * bytes;
         C *c = reinterpret_cast<C *>(base);
@@ -658,6 +801,37 @@ struct SlotWork {
         double2 *d = reinterpret_cast<double2 *>(base + 5 * NN * sizeof(C));
         es = d;
         et = d + NN;
+    }
+};
+
+// Per-slot workspace beyond order 80: alpha and beta twice (current and next,
+// swapped every iteration: the blocked GEMM reads B over several passes, so it
+// cannot overwrite it), m1, m2, g in precision R, eps_s and eps_t in FP64, and
+// the Gauss-Jordan matrix (NP x (NP+1), FP64-sized), all in global memory.
+template <int Q, typename R>
+struct SlotWorkL {
+    using C = typename Prec<R>::C;
+    static constexpr size_t NN = (size_t)(16 * Q) * (16 * Q);
+    static constexpr size_t SB = (size_t)(16 * Q) * (16 * Q + 1) * sizeof(double2);
+    static constexpr size_t bytes = 7 * NN * sizeof(C) + 2 * NN * sizeof(double2) + SB;
+    C *ab, *m1, *m2, *g;  // ab: alpha[0], alpha[1], beta[0], beta[1]
+    double2 *es, *et;
+    unsigned char *S;
+    __device__ C *alpha(int par) const { return ab + par * NN; }
+    __device__ C *beta(int par) const { return ab + (2 + par) * NN; }
+    __device__ C *alpha_next(int par) const { return ab + (par ^ 1) * NN; }
+    __device__ C *beta_next(int par) const { return ab + (2 + (par ^ 1)) * NN; }
+    __device__ SlotWorkL(void *work, int slot) {
+        unsigned char *base = reinterpret_cast<unsigned char *>(work) + (size_t)slot * bytes;
+        C *c = reinterpret_cast<C *>(base);
+        ab = c;
+        m1 = c + 4 * NN;
+        m2 = c + 5 * NN;
+        g = c + 6 * NN;
+        double2 *d = reinterpret_cast<double2 *>(base + 7 * NN * sizeof(C));
+        es = d;
+        et = d + NN;
+        S = reinterpret_cast<unsigned char *>(d + 2 * NN);
     }
 };
 
@@ -674,27 +848,21 @@ __device__ __forceinline__ void instance_of(const DecimateArgs &a, int t, int &i
 }
 
 // One Gauss-Jordan per slot: a slot without an instance takes the next one
-// (eps_s = eps_t = H00, alpha = H01, beta = H01^dagger) and starts
-// its first iteration; kLoop computes g = (z - eps_i)^-1 for the GEMM round;
-// kFinal0..2 compute the diagonals of GLL = (z - eps_s)^-1,
-// GRR = (z - eps_t)^-1, GB = (z - eps_i)^-1 in FP64 and their traces, and the
-// last of them writes the instance's outputs.
-template <int Q, typename R>
-__global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int slot0) {
+// (eps_s = eps_t = H00, alpha = H01, beta = H01^dagger) and starts its first
+// iteration; kLoop computes g = (z - eps_i)^-1 for the GEMM round; kFinal0..2
+// compute the diagonals of GLL = (z - eps_s)^-1, GRR = (z - eps_t)^-1,
+// GB = (z - eps_i)^-1 in FP64 and their traces, and the last of them writes
+// the instance's outputs.  sR / tR hold the matrix and the panel pivot rows
+// for the precision-R inverse, s64 / t64 for the FP64 final ones.
+template <int Q, typename R, int NT, typename W>
+__device__ __forceinline__ void gj_round(const DecimateArgs &a, int slot, const W &w,
+                                         Common<Q> &cm, unsigned char *sR, unsigned char *tR,
+                                         unsigned char *s64, unsigned char *t64) {
     using C = typename Prec<R>::C;
     constexpr int NP = 16 * Q;
     constexpr int NN = NP * NP;
-    // The matrix region is dynamic shared memory (GjLay<Q, R>::bytes at
-    // launch): with its size unknown here, the compiler does not take the
-    // one-workgroup-per-CU occupancy it implies as licence to claim every
-    // register, and the kernel's register count stays what it uses, small
-    // enough to leave room for a GEMM workgroup beside it.
-    extern __shared__ __align__(16) unsigned char lds[];
-    __shared__ Common<Q> cm;
     const int tid = threadIdx.x;
-    const int slot = slot0 + blockIdx.x;
     SlotState *st = a.state + slot;
-    const SlotWork<Q, R> w(a.work, slot);
     const int n = a.n;
 
     if (tid == 0) cm.s = *st;
@@ -714,7 +882,8 @@ __global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int 
         instance_of(a, t, ik, z);
         const double2 *H00 = a.hk + (size_t)ik * 2 * NN;
         const double2 *H01 = H00 + NN;
-        for (int e = tid; e < NN; e += kGjBlock) {
+        C *al = w.alpha(0), *be = w.beta(0);
+        for (int e = tid; e < NN; e += NT) {
             const double2 h = H00[e];
             w.es[e] = h;
             w.et[e] = h;
@@ -722,13 +891,13 @@ __global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int 
             C av;
             av.x = (R)h1.x;
             av.y = (R)h1.y;
-            w.al[e] = av;
+            al[e] = av;
             const int i = e % NP, j = e / NP;
             const double2 ht = H01[j + i * NP];
             C bv;
             bv.x = (R)ht.x;
             bv.y = (R)(-ht.y);
-            w.be[e] = bv;
+            be[e] = bv;
         }
         if (tid == 0) {
             cm.s.it = 1;
@@ -744,8 +913,7 @@ __global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int 
     instance_of(a, uniform(cm.s.t), ik, z);
     const double2 *H00 = a.hk + (size_t)ik * 2 * NN;
     if (phase == kLoop) {
-        const bool sing =
-            gj_inverse<Q, R, kGjBlock>(lds, cm, w.es, w.et, H00, z, n, 4, nullptr, w.g);
+        const bool sing = gj_inverse<Q, R, NT>(sR, tR, cm, w.es, w.et, H00, z, n, 4, nullptr, w.g);
         if (tid == 0) {
             if (sing) cm.s.status |= 1;
             cm.s.phase = kGemm;
@@ -756,8 +924,8 @@ __global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int 
         // eps_s, eps_t, eps_i = eps_s + eps_t - H00
         const double2 *eps = q == 1 ? w.et : w.es;
         const double2 *eps2 = q == 2 ? w.et : nullptr;
-        const bool sing = gj_inverse<Q, double, kGjBlock>(lds, cm, eps, eps2, H00, z, n,
-                                                          full ? 2 : 1, full, nullptr);
+        const bool sing = gj_inverse<Q, double, NT>(s64, t64, cm, eps, eps2, H00, z, n,
+                                                    full ? 2 : 1, full, nullptr);
         if (tid == 0) {
             if (sing) cm.s.status |= 1;
             if (q == 0) {
@@ -792,6 +960,33 @@ __global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int 
         }
     }
     if (tid == 0) *st = cm.s;
+}
+
+// Orders up to 80: the matrix in LDS, 4 waves (one per SIMD).
+template <int Q, typename R>
+__global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int slot0) {
+    // The matrix region is dynamic shared memory (GjLay<Q, R>::bytes at
+    // launch): with its size unknown here, the compiler does not take the
+    // one-workgroup-per-CU occupancy it implies as licence to claim every
+    // register, and the kernel's register count stays what it uses, small
+    // enough to leave room for a GEMM workgroup beside it.
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ Common<Q> cm;
+    const int slot = slot0 + blockIdx.x;
+    const SlotWork<Q, R> w(a.work, slot);
+    gj_round<Q, R, kGjBlock>(a, slot, w, cm, lds, lds + GjLay<Q, R>::S, lds,
+                             lds + GjLay<Q, double>::S);
+}
+
+// Orders beyond 80: the matrix in the slot's global workspace (it no longer
+// fits in LDS), the panel pivot rows in LDS, 8 waves.
+template <int Q, typename R>
+__global__ void __launch_bounds__(kBlock) gj_large_round_kernel(DecimateArgs a, int slot0) {
+    __shared__ __align__(16) unsigned char tlds[(size_t)kPanel * 16 * Q * sizeof(double2)];
+    __shared__ Common<Q> cm;
+    const int slot = slot0 + blockIdx.x;
+    const SlotWorkL<Q, R> w(a.work, slot);
+    gj_round<Q, R, kBlock>(a, slot, w, cm, w.S, tlds, w.S, tlds);
 }
 
 // The three dual GEMMs of one iteration for every slot in kGemm, then the
@@ -831,32 +1026,84 @@ gemm_round_kernel(DecimateArgs a, int slot0) {
     }
 }
 
-constexpr int kMaxQ = 5;
+// The same for orders beyond 80, with the blocked GEMM; alpha' and beta' go to
+// the other half of the slot's alpha / beta pair (iteration parity).
+template <int Q, typename R>
+__global__ void __launch_bounds__(kBlock) gemm_large_round_kernel(DecimateArgs a, int slot0) {
+    __shared__ __align__(16) unsigned char lds[GemmLayL<Q, R>::bytes];
+    __shared__ double red[kWaves];
+    __shared__ SlotState sst;
+    const int slot = slot0 + blockIdx.x;
+    SlotState *st = a.state + slot;
+    if (threadIdx.x == 0) sst = *st;
+    __syncthreads();
+    if (uniform(sst.phase) != kGemm) return;
+    const int par = uniform((sst.it - 1) & 1);
+    const SlotWorkL<Q, R> w(a.work, slot);
+    const int n = a.n;
+    dual_gemm_blocked<Q, R, 0>(lds, red, w.alpha(par), w.beta(par), w.g, n, w.m1, w.m2, nullptr);
+    __syncthreads();
+    dual_gemm_blocked<Q, R, 1>(lds, red, w.m1, w.m2, w.beta(par), n, nullptr, w.beta_next(par), w.es);
+    __syncthreads();
+    const double s = dual_gemm_blocked<Q, R, 2>(lds, red, w.m2, w.m1, w.alpha(par), n, nullptr,
+                                                 w.alpha_next(par), w.et);
+    if (threadIdx.x == 0) {
+        SlotState v = sst;
+        if (s <= a.accuracy) {
+            v.phase = kFinal0;
+        } else if (v.it >= a.itermax) {
+            v.status |= 2;
+            v.phase = kFinal0;
+        } else {
+            v.it += 1;
+            v.phase = kLoop;
+        }
+        *st = v;
+    }
+}
+
+constexpr int kMaxQ = 18;
 constexpr int kRoundsPerBatch = 8;
 
-int padded_q(int n) { return (n + 15) / 16; }
+// Orders are padded to 16 Q; kernels exist for Q = 1..5 (matrix in LDS) and
+// for even Q = 6..18 beyond (an odd Q there takes the next even one: the pad
+// rows and columns are exact zeros).
+int padded_q(int n) {
+    const int q = (n + 15) / 16;
+    return q <= 5 ? q : q + (q & 1);
+}
 int decimate_max_q() { return kMaxQ; }
 
 template <int Q, typename R>
 static int launch_round(const DecimateArgs &a, int per_set, int set, bool gj, hipStream_t s) {
-    constexpr size_t gj_lds = GjLay<Q, R>::bytes;
-    if (gj)
-        hipLaunchKernelGGL((gj_round_kernel<Q, R>), dim3(per_set), dim3(kGjBlock), gj_lds, s, a,
-                           set * per_set);
-    else
-        hipLaunchKernelGGL((gemm_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), 0, s, a,
-                           set * per_set);
+    if constexpr (Q <= 5) {
+        constexpr size_t gj_lds = GjLay<Q, R>::bytes;
+        if (gj)
+            hipLaunchKernelGGL((gj_round_kernel<Q, R>), dim3(per_set), dim3(kGjBlock), gj_lds, s, a,
+                               set * per_set);
+        else
+            hipLaunchKernelGGL((gemm_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), 0, s, a,
+                               set * per_set);
+    } else {
+        if (gj)
+            hipLaunchKernelGGL((gj_large_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), 0, s, a,
+                               set * per_set);
+        else
+            hipLaunchKernelGGL((gemm_large_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), 0, s, a,
+                               set * per_set);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+#define WTSURF_FOR_EACH_Q(X) X(1) X(2) X(3) X(4) X(5) X(6) X(8) X(10) X(12) X(14) X(16) X(18)
 
 template <typename R>
 static int dispatch_round(const DecimateArgs &a, int q, int per_set, int set, bool gj, hipStream_t s) {
     switch (q) {
-        case 1: return launch_round<1, R>(a, per_set, set, gj, s);
-        case 2: return launch_round<2, R>(a, per_set, set, gj, s);
-        case 3: return launch_round<3, R>(a, per_set, set, gj, s);
-        case 4: return launch_round<4, R>(a, per_set, set, gj, s);
-        case 5: return launch_round<5, R>(a, per_set, set, gj, s);
+#define WTSURF_CASE(Q) \
+    case Q: return launch_round<Q, R>(a, per_set, set, gj, s);
+        WTSURF_FOR_EACH_Q(WTSURF_CASE)
+#undef WTSURF_CASE
         default: return -1;
     }
 }
@@ -864,11 +1111,10 @@ static int dispatch_round(const DecimateArgs &a, int q, int per_set, int set, bo
 template <typename R>
 static size_t dispatch_slot_bytes(int q) {
     switch (q) {
-        case 1: return SlotWork<1, R>::bytes;
-        case 2: return SlotWork<2, R>::bytes;
-        case 3: return SlotWork<3, R>::bytes;
-        case 4: return SlotWork<4, R>::bytes;
-        case 5: return SlotWork<5, R>::bytes;
+#define WTSURF_CASE(Q) \
+    case Q: return Q <= 5 ? SlotWork<Q, R>::bytes : SlotWorkL<Q, R>::bytes;
+        WTSURF_FOR_EACH_Q(WTSURF_CASE)
+#undef WTSURF_CASE
         default: return 0;
     }
 }
@@ -912,7 +1158,7 @@ static int launch_phase(const DecimateArgs &a, int q, int precision, int per_set
 }
 
 int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const RoundCtl &ctl) {
-    hipStream_t s0 = ctl.s[0], s1 = ctl.s[1];
+    hipStream_t s0 = ctl.s[0], s1 = ctl.s[1], sc = ctl.s[2];
     if (precision == 0)
         dispatch_allow_lds<double>(q);
     else
@@ -925,13 +1171,16 @@ int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const
     const long max_batches = per_slot * (a.itermax + 4) / kRoundsPerBatch + 4;
     if (hipMemsetAsync(a.state, 0, (size_t)2 * per_set * sizeof(SlotState), s0) != hipSuccess ||
         hipMemsetAsync(a.queue, 0, 2 * sizeof(unsigned), s0) != hipSuccess ||
-        hipEventRecord(ctl.ev[2], s0) != hipSuccess || hipStreamWaitEvent(s1, ctl.ev[2], 0) != hipSuccess)
+        hipEventRecord(ctl.ev[6], s0) != hipSuccess || hipStreamWaitEvent(s1, ctl.ev[6], 0) != hipSuccess)
         return -1;
     // Stream 0 runs set 0 as (inverse, GEMMs) rounds, stream 1 runs set 1 as
-    // (GEMMs, inverse): the sets stay half a round apart.  Every batch of
-    // rounds ends with a copy of the finished-instance count; the count of
-    // the batch before is read while this one runs, so the device never
-    // waits for the host.
+    // (GEMMs, inverse): the sets stay half a round apart.  Each stream marks
+    // the end of every batch of rounds with an event; a third stream waits for
+    // both marks of a batch and copies the finished-instance count to pinned
+    // host memory, so the count it reports covers whole batches of both sets
+    // whichever stream runs ahead, and the two work streams never wait for
+    // each other.  The host reads the count of the batch before while this one
+    // runs, so the device never waits for the host.
     for (long b = 0;; ++b) {
         if (b > max_batches) return -2;
         for (int r = 0; r < kRoundsPerBatch; ++r) {
@@ -941,16 +1190,21 @@ int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const
                 launch_phase(a, q, precision, per_set, 1, true, s1))
                 return -1;
         }
-        if (hipMemcpyAsync(ctl.pinned + (b & 1), a.queue + 1, sizeof(unsigned),
-                           hipMemcpyDeviceToHost, s0) != hipSuccess ||
-            hipEventRecord(ctl.ev[b & 1], s0) != hipSuccess)
+        const int h = (int)(b & 1);
+        if (hipEventRecord(ctl.ev[h], s0) != hipSuccess ||
+            hipEventRecord(ctl.ev[2 + h], s1) != hipSuccess ||
+            hipStreamWaitEvent(sc, ctl.ev[h], 0) != hipSuccess ||
+            hipStreamWaitEvent(sc, ctl.ev[2 + h], 0) != hipSuccess ||
+            hipMemcpyAsync(ctl.pinned + h, a.queue + 1, sizeof(unsigned), hipMemcpyDeviceToHost, sc) !=
+                hipSuccess ||
+            hipEventRecord(ctl.ev[4 + h], sc) != hipSuccess)
             return -1;
         if (b > 0) {
-            if (hipEventSynchronize(ctl.ev[(b - 1) & 1]) != hipSuccess) return -1;
-            if (ctl.pinned[(b - 1) & 1] >= (unsigned)a.ninst) break;
+            if (hipEventSynchronize(ctl.ev[4 + (h ^ 1)]) != hipSuccess) return -1;
+            if (ctl.pinned[h ^ 1] >= (unsigned)a.ninst) break;
         }
     }
-    if (hipEventRecord(ctl.ev[2], s1) != hipSuccess || hipStreamWaitEvent(s0, ctl.ev[2], 0) != hipSuccess)
+    if (hipEventRecord(ctl.ev[6], s1) != hipSuccess || hipStreamWaitEvent(s0, ctl.ev[6], 0) != hipSuccess)
         return -1;
     return 0;
 }

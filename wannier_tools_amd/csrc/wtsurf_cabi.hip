@@ -50,8 +50,8 @@ struct wtsurf_ctx {
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     // round driver of the decimation: a second stream, its events and a
     // pinned progress counter
-    hipStream_t aux = nullptr;
-    hipEvent_t drv[3] = {nullptr, nullptr, nullptr};
+    hipStream_t aux = nullptr, cnt = nullptr;
+    hipEvent_t drv[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     unsigned *pinned = nullptr;
     std::string err;
 
@@ -155,6 +155,7 @@ int wtsurf_ctx_create(int device, wtsurf_ctx **out) {
     c->num_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->cnt, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&c->pinned, 2 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess) {
         wtsurf_ctx_destroy(c);
         return -2;
@@ -175,6 +176,7 @@ void wtsurf_ctx_destroy(wtsurf_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
+    if (c->cnt) (void)hipStreamSynchronize(c->cnt);
     c->hr.release(); c->irvec.release(); c->slot_begin.release();
     c->d_top.release(); c->d_bot.release();
     c->kpts.release(); c->omega.release(); c->phase.release(); c->hk.release();
@@ -188,6 +190,7 @@ void wtsurf_ctx_destroy(wtsurf_ctx *c) {
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->aux) (void)hipStreamDestroy(c->aux);
+    if (c->cnt) (void)hipStreamDestroy(c->cnt);
     delete c;
 }
 
@@ -332,10 +335,12 @@ static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const
     RoundCtl ctl{};
     ctl.s[0] = c->stream;
     ctl.s[1] = c->aux;
-    for (int i = 0; i < 3; ++i) ctl.ev[i] = c->drv[i];
+    ctl.s[2] = c->cnt;
+    for (int i = 0; i < 7; ++i) ctl.ev[i] = c->drv[i];
     ctl.pinned = c->pinned;
     if (int r = decimate_run(a, q, precision, c->per_set, ctl)) {
-        (void)hipStreamSynchronize(c->aux);  // rounds already queued on the second stream
+        (void)hipStreamSynchronize(c->aux);  // rounds already queued on the other streams
+        (void)hipStreamSynchronize(c->cnt);
         if (r == -2) return c->fail("decimation made no progress within its round bound");
         const hipError_t e = hipGetLastError();
         return e != hipSuccess ? c->hipfail(e, "decimation rounds") : c->fail("decimation rounds failed");

@@ -44,11 +44,12 @@ struct DecimateArgs {
     double2 *gfull;         // [ninst][3][n*n] GLL, GRR, GB or null
 };
 
-// Streams and host-visible progress counter of the round driver: two
-// streams at default priority, three events, two pinned unsigneds.
+// Streams and host-visible progress counter of the round driver: two work
+// streams and a counter-copy stream, all at default priority; seven events;
+// two pinned unsigneds.
 struct RoundCtl {
-    hipStream_t s[2];
-    hipEvent_t ev[3];
+    hipStream_t s[3];
+    hipEvent_t ev[7];
     unsigned *pinned;
 };
 
@@ -61,6 +62,7 @@ size_t decimate_slot_bytes(int q, int precision);
 // Enqueues the decimation of a.ninst instances on ctl.s[0] and ctl.s[1] and
 // returns once every instance is counted finished (the work is complete when
 // ctl.s[0] reaches the end of what was enqueued; ctl.s[0] waits for s[1]).
+// Returns -2 when the count stops short of a.ninst within the round bound.
 int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const RoundCtl &ctl);
 
 // Fourier sum H00(k), H01(k) (src/ham_qlayer2qlayer.f90:45-97).
