@@ -34,8 +34,8 @@ enum {
     WTSURF_STATUS_NOT_CONVERGED = 2  /* itermax reached (src/surfgreen.f90:86) */
 };
 
-/* Context on HIP device `device` with its own two streams (default
- * priority).  No reference counterpart (the reference has no device state). */
+/* Context on HIP device `device` with its own streams (default priority).
+ * No reference counterpart (the reference has no device state). */
 int wtsurf_ctx_create(int device, wtsurf_ctx **ctx);
 void wtsurf_ctx_destroy(wtsurf_ctx *ctx);
 /* Message of the last failed call on ctx ("" if none; never NULL). */
@@ -87,6 +87,18 @@ int wtsurf_spectral_grid(wtsurf_ctx *ctx, int nk, const double *kpts,
                          double divisor, int precision, double *dos_l,
                          double *dos_r, double *dos_bulk, int *iters,
                          int *status);
+
+/* Per-instance batch (the mixed workload: any set of (k, omega, eta)):
+ * instance i is k = kpts[k_index[i]] (k_index 0-based, < nk), omega[i] and
+ * eta[i]; H00/H01 are formed once per k point.  Traces as
+ * wtsurf_spectral_grid, in instance order ([ninst]); iters and status may be
+ * NULL.  The reference drives the same solves from its (k, omega) loops
+ * (src/surfstat.f90:123-160, src/fermiarc.f90:193-248); this entry point has no
+ * single counterpart there. */
+int wtsurf_spectral_batch(wtsurf_ctx *ctx, int nk, const double *kpts, int ninst,
+                          const int *k_index, const double *omega, const double *eta,
+                          double divisor, int precision, double *dos_l,
+                          double *dos_r, double *dos_bulk, int *iters, int *status);
 
 /* The same computation split so inputs can stay resident in device memory:
  * upload the grid once, run, wait, fetch.  The run enqueues its work on the

@@ -73,6 +73,9 @@ struct wtsurf_ctx {
     DevBuf<double2> phase, hk;
     DevBuf<double> dos_l, dos_r, dos_b;
     DevBuf<int> iters, status;
+    // per-instance batch (wtsurf_spectral_batch)
+    DevBuf<int> inst_k;
+    DevBuf<double> inst_w, inst_eta;
     // decimation workspace
     DevBuf<unsigned char> work, state;
     DevBuf<unsigned> queue;
@@ -181,6 +184,7 @@ void wtsurf_ctx_destroy(wtsurf_ctx *c) {
     c->d_top.release(); c->d_bot.release();
     c->kpts.release(); c->omega.release(); c->phase.release(); c->hk.release();
     c->dos_l.release(); c->dos_r.release(); c->dos_b.release();
+    c->inst_k.release(); c->inst_w.release(); c->inst_eta.release();
     c->iters.release(); c->status.release(); c->work.release(); c->state.release();
     c->queue.release();
     for (auto &e : c->ev)
@@ -314,10 +318,17 @@ int wtsurf_ham_qlayer2qlayer(wtsurf_ctx *c, int nk, const double *kpts, double *
     return 0;
 }
 
+// Per-instance (k index, omega, eta) device arrays for a batch run, or all null
+// for the k x omega grid with one eta.
+struct InstanceLists {
+    const int *k = nullptr;
+    const double *w = nullptr, *eta = nullptr;
+};
+
 static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const double *d_omega,
                              double eta, double divisor, double2 *gfull, int n, int nw,
                              const int *d_top, int ntop, const int *d_bot, int nbot,
-                             int precision) {
+                             int precision, InstanceLists inst = InstanceLists()) {
     const int q = padded_q(n);
     if (q > decimate_max_q()) return c->fail("Ndim exceeds the largest built decimation kernel");
     if (int r = ensure_work(c, q, precision, ninst)) return r;
@@ -325,7 +336,7 @@ static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const
         c->iters.ensure(ninst) || c->status.ensure(ninst))
         return c->fail("out of device memory (outputs)");
     DecimateArgs a{};
-    a.hk = c->hk.p; a.omega = d_omega; a.eta = nullptr; a.inst_k = nullptr; a.inst_w = nullptr;
+    a.hk = c->hk.p; a.omega = d_omega; a.eta = inst.eta; a.inst_k = inst.k; a.inst_w = inst.w;
     a.eta0 = eta; a.nk = nk; a.nomega = nomega; a.ninst = ninst; a.n = n; a.nw = nw;
     a.itermax = c->itermax; a.accuracy = c->accuracy;
     a.top = d_top; a.ntop = ntop; a.bot = d_bot; a.nbot = nbot; a.divisor = divisor;
@@ -465,6 +476,49 @@ int wtsurf_grid_download(wtsurf_ctx *c, double *dos_l, double *dos_r, double *do
     if (dos_l) WT_HIP(c, hipMemcpy(dos_l, c->dos_l.p, ni * sizeof(double), hipMemcpyDeviceToHost));
     if (dos_r) WT_HIP(c, hipMemcpy(dos_r, c->dos_r.p, ni * sizeof(double), hipMemcpyDeviceToHost));
     if (dos_bulk) WT_HIP(c, hipMemcpy(dos_bulk, c->dos_b.p, ni * sizeof(double), hipMemcpyDeviceToHost));
+    if (iters) WT_HIP(c, hipMemcpy(iters, c->iters.p, ni * sizeof(int), hipMemcpyDeviceToHost));
+    if (status) WT_HIP(c, hipMemcpy(status, c->status.p, ni * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int wtsurf_spectral_batch(wtsurf_ctx *c, int nk, const double *kpts, int ninst, const int *k_index,
+                          const double *omega, const double *eta, double divisor, int precision,
+                          double *dos_l, double *dos_r, double *dos_bulk, int *iters, int *status) {
+    if (!c) return -1;
+    if (!c->has_model) return c->fail("no model (wtsurf_set_model)");
+    if (nk <= 0 || ninst <= 0 || !kpts || !k_index || !omega || !eta || !dos_l || !dos_r || !dos_bulk)
+        return c->fail("invalid arguments");
+    if (precision != WTSURF_FP64 && precision != WTSURF_FP32_REFINE)
+        return c->fail("unknown precision mode");
+    if (!(divisor != 0.0)) return c->fail("divisor must be non-zero");
+    for (int i = 0; i < ninst; ++i)
+        if (k_index[i] < 0 || k_index[i] >= nk) return c->fail("k index outside 0..nk-1");
+    if (int r = use_device(c)) return r;
+    WT_HIP(c, hipStreamSynchronize(c->stream));  // a run in flight may still read the buffers
+    if (c->kpts.ensure(2 * (size_t)nk) || c->inst_k.ensure(ninst) || c->inst_w.ensure(ninst) ||
+        c->inst_eta.ensure(ninst))
+        return c->fail("out of device memory");
+    WT_HIP(c, hipMemcpy(c->kpts.p, kpts, 2 * (size_t)nk * sizeof(double), hipMemcpyHostToDevice));
+    WT_HIP(c, hipMemcpy(c->inst_k.p, k_index, (size_t)ninst * sizeof(int), hipMemcpyHostToDevice));
+    WT_HIP(c, hipMemcpy(c->inst_w.p, omega, (size_t)ninst * sizeof(double), hipMemcpyHostToDevice));
+    WT_HIP(c, hipMemcpy(c->inst_eta.p, eta, (size_t)ninst * sizeof(double), hipMemcpyHostToDevice));
+    c->nk = nk;
+    c->nomega = 0;  // the grid no longer matches; wtsurf_grid_upload again for a grid run
+    c->ran = false;
+    if (int r = run_fourier(c)) return r;
+    InstanceLists inst;
+    inst.k = c->inst_k.p;
+    inst.w = c->inst_w.p;
+    inst.eta = c->inst_eta.p;
+    if (int r = launch_decimation(c, ninst, nk, 1, nullptr, 0.0, divisor, nullptr, c->n, c->nw,
+                                  c->d_top.p, (int)c->top.size(), c->d_bot.p, (int)c->bot.size(),
+                                  precision, inst))
+        return r;
+    WT_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t ni = (size_t)ninst;
+    WT_HIP(c, hipMemcpy(dos_l, c->dos_l.p, ni * sizeof(double), hipMemcpyDeviceToHost));
+    WT_HIP(c, hipMemcpy(dos_r, c->dos_r.p, ni * sizeof(double), hipMemcpyDeviceToHost));
+    WT_HIP(c, hipMemcpy(dos_bulk, c->dos_b.p, ni * sizeof(double), hipMemcpyDeviceToHost));
     if (iters) WT_HIP(c, hipMemcpy(iters, c->iters.p, ni * sizeof(int), hipMemcpyDeviceToHost));
     if (status) WT_HIP(c, hipMemcpy(status, c->status.p, ni * sizeof(int), hipMemcpyDeviceToHost));
     return 0;

@@ -50,6 +50,9 @@ SIGNATURES = {
     "wtsurf_spectral_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _dp,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                             _dp, _dp, _dp, _ip, _ip]),
+    "wtsurf_spectral_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _ip,
+                                             _dp, _dp, ctypes.c_double, ctypes.c_int,
+                                             _dp, _dp, _dp, _ip, _ip]),
     "wtsurf_grid_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_int, _dp]),
     "wtsurf_grid_run_async": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_int]),
@@ -199,6 +202,30 @@ class SurfaceGF:
         self.run_async(eta, divisor, precision)
         self.sync()
         return self.download()
+
+    def spectral_batch(self, kpts, k_index, omegas, etas, divisor: float = 1.0,
+                       precision="fp64"):
+        """Per-instance (k_index[i] into kpts, omegas[i], etas[i]): dos_l, dos_r,
+        dos_bulk, iterations, status, each of shape (ninst,)."""
+        if precision not in _PRECISION:
+            raise ValueError(f"unknown precision {precision!r}")
+        k = np.ascontiguousarray(kpts, dtype=np.float64).reshape(-1, 2)
+        ki = np.ascontiguousarray(k_index, dtype=np.int32).ravel()
+        w = np.ascontiguousarray(omegas, dtype=np.float64).ravel()
+        e = np.ascontiguousarray(etas, dtype=np.float64).ravel()
+        if not (ki.size == w.size == e.size):
+            raise ValueError("k_index, omegas and etas must have one entry per instance")
+        out = [np.empty(ki.size) for _ in range(3)]
+        it = np.empty(ki.size, dtype=np.int32)
+        st = np.empty(ki.size, dtype=np.int32)
+        self._check(self.lib.wtsurf_spectral_batch(self.ctx, k.shape[0], _dptr(k), ki.size, _iptr(ki),
+                                                   _dptr(w), _dptr(e), float(divisor),
+                                                   _PRECISION[precision], _dptr(out[0]),
+                                                   _dptr(out[1]), _dptr(out[2]), _iptr(it),
+                                                   _iptr(st)),
+                    "wtsurf_spectral_batch")
+        self._grid = (0, 0)
+        return out[0], out[1], out[2], it, st
 
     # ---- resident split (inputs stay in device memory) -------------------
     def upload_grid(self, kpts, omegas) -> None:
