@@ -116,6 +116,30 @@ int wtsurf_grid_timing(wtsurf_ctx *ctx, float *ms_fourier, float *ms_decimate);
 int wtsurf_grid_download(wtsurf_ctx *ctx, double *dos_l, double *dos_r,
                          double *dos_bulk, int *iters, int *status);
 
+/* Spin-resolved traces, taken by the runs that follow (wtsurf_spectral_grid,
+ * wtsurf_grid_run_async, wtsurf_spectral_batch) from the same GLL and GRR:
+ *   s_c = -sum_{io} Im (G sigma_c)(io, io) / divisor,   c = x, y, z,
+ * with the spin operators of src/surfstat.f90:102-115 (orbitals 1..Num_wann/2
+ * of every principal layer spin up, the next Num_wann/2 their spin-down
+ * partners).  The left sums run over the top orbitals on GLL.  The right sums
+ * run on GRR over
+ *   WTSURF_SPIN_SURFSTAT: the top orbital indices, as surfstat does
+ *                         (src/surfstat.f90:178-191);
+ *   WTSURF_SPIN_ARC:      the bottom orbitals of the last principal layer, as
+ *                         SurfaceDOSkk does (src/fermiarc.f90:298-327).
+ * The reference forms each sum from a full ZGEMM G*sigma (src/mat_mul.f90:3-27);
+ * here each term is one element of G, the same number. */
+enum {
+    WTSURF_SPIN_OFF = 0,
+    WTSURF_SPIN_SURFSTAT = 1,
+    WTSURF_SPIN_ARC = 2
+};
+int wtsurf_set_spin_traces(wtsurf_ctx *ctx, int mode);
+/* The last run's spin traces (fails if it took none): spin_l and spin_r are
+ * [3][ninst] each (sx, sy, sz blocks), every block in the order of that run's
+ * dos_l.  Either pointer may be NULL. */
+int wtsurf_spin_download(wtsurf_ctx *ctx, double *spin_l, double *spin_r);
+
 /* Largest Ndim the FP64 decimation kernels are built for. */
 int wtsurf_max_ndim(void);
 

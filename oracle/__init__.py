@@ -66,6 +66,7 @@ class Oracle:
                                      ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, _ip,
                                      ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _ip])
+        lib.wto_spectral_spin.argtypes = lib.wto_spectral.argtypes + [ctypes.c_int, _dp, _dp]
         self.lib = lib
 
     @staticmethod
@@ -140,6 +141,37 @@ class Oracle:
         if rc < 0:
             raise RuntimeError(f"wto_spectral failed ({rc})")
         return dl, dr, db, its
+
+    def spectral_spin(self, nw, np_, irvec, hr, kpts, omegas, eta, top, bot, spin_mode,
+                      ijmax=10, itermax=100, accuracy=None, divisor=1.0, nthreads=0):
+        """spectral() plus the spin-resolved traces: returns dl, dr, db, iters,
+        spin_l, spin_r with spin_* of shape (3, nk, nomega) (sx, sy, sz).
+        spin_mode 1: right sums over top on GRR (surfstat); 2: over bot in the
+        last layer (SurfaceDOSkk)."""
+        from wannier_tools_amd.model import DECIMATION_ACCURACY
+        acc = DECIMATION_ACCURACY if accuracy is None else accuracy
+        irvec, hr_cm = self._model_arrays(irvec, hr)
+        kpts = np.ascontiguousarray(kpts, dtype=np.float64).reshape(-1, 2)
+        omegas = np.ascontiguousarray(omegas, dtype=np.float64).ravel()
+        top = np.ascontiguousarray(top, dtype=np.int32)
+        bot = np.ascontiguousarray(bot, dtype=np.int32)
+        nk, nom = kpts.shape[0], omegas.shape[0]
+        dl = np.zeros((nk, nom), dtype=np.float64, order="F")
+        dr = np.zeros_like(dl)
+        db = np.zeros_like(dl)
+        its = np.zeros((nk, nom), dtype=np.int32, order="F")
+        sl = np.zeros((nk, nom, 3), dtype=np.float64, order="F")
+        sr = np.zeros_like(sl)
+        rc = self.lib.wto_spectral_spin(nw, np_, irvec.shape[0], ijmax, _i(irvec),
+                                        _d(hr_cm.view(np.float64)), nk, _d(kpts), nom,
+                                        _d(omegas), eta, top.size, _i(top), bot.size, _i(bot),
+                                        itermax, acc, divisor, nthreads, dl.ctypes.data_as(_dp),
+                                        dr.ctypes.data_as(_dp), db.ctypes.data_as(_dp),
+                                        its.ctypes.data_as(_ip), spin_mode,
+                                        sl.ctypes.data_as(_dp), sr.ctypes.data_as(_dp))
+        if rc < 0:
+            raise RuntimeError(f"wto_spectral_spin failed ({rc})")
+        return dl, dr, db, its, np.moveaxis(sl, 2, 0), np.moveaxis(sr, 2, 0)
 
 
 class RefHarness:

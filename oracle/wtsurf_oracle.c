@@ -232,23 +232,71 @@ int wto_surfgreen_1985(int n, double omega, double eta, const double *h00,
     return info;
 }
 
-/* src/surfstat.f90:123-160 (divisor = 1) and src/fermiarc.f90:206-248
+/* Spin operators of src/surfstat.f90:102-115 (the same as
+ * src/fermiarc.f90:85-101): in principal layer i (0-based here) and for
+ * j < nw/2, sigma_x(a, b) = sigma_x(b, a) = 1, sigma_y(a, b) = -i,
+ * sigma_y(b, a) = i, sigma_z(a, a) = 1, sigma_z(b, b) = -1 with a = nw i + j,
+ * b = a + nw/2.  s is three n x n matrices (x, y, z), zeroed here. */
+static void spin_operators(int nw, int np, cplx *s)
+{
+    const int n = nw * np, nh = nw / 2;
+    const size_t nn = (size_t)n * n;
+    memset(s, 0, sizeof(cplx) * 3 * nn);
+    cplx *sx = s, *sy = s + nn, *sz = s + 2 * nn;
+    for (int i = 0; i < np; ++i)
+        for (int j = 0; j < nh; ++j) {
+            const size_t a = (size_t)nw * i + j, b = a + nh;
+            sx[a + n * b] = 1.0;
+            sx[b + n * a] = 1.0;
+            sy[a + n * b] = -I;
+            sy[b + n * a] = I;
+            sz[a + n * a] = 1.0;
+            sz[b + n * b] = -1.0;
+        }
+}
+
+/* -sum_{e} Im (G sigma_c)(io_e, io_e) / divisor for c = x, y, z, with the full
+ * product G sigma_c formed as the reference does (mat_mul, src/surfstat.f90:163-192;
+ * src/fermiarc.f90:270-327). */
+static void spin_sums(int n, const cplx *g, const cplx *sig, cplx *ctemp, const int *list,
+                      int cnt, int offset, double divisor, double out[3])
+{
+    const size_t nn = (size_t)n * n;
+    for (int c = 0; c < 3; ++c) {
+        gemm(n, g, sig + c * nn, ctemp);
+        double s = 0.0;
+        for (int e = 0; e < cnt; ++e) {
+            const int io = offset + list[e] - 1;
+            s -= cimag(ctemp[io + (size_t)n * io]) / divisor;
+        }
+        out[c] = s;
+    }
+}
+
+/* src/surfstat.f90:123-192 (divisor = 1) and src/fermiarc.f90:206-328
  * (divisor = pi): dos_l = -sum_{io in top} Im GLL(io,io),
- * dos_r = -sum_{b in bot} Im GRR(N-nw+b, N-nw+b), dos_bulk = -sum_i Im GB(i,i). */
-int wto_spectral(int nw, int np, int nr, int ijmax, const int *irvec,
-                 const double *hr, int nk, const double *kpts, int nomega,
-                 const double *omega, double eta, int ntop, const int *top,
-                 int nbot, const int *bot, int itermax, double accuracy,
-                 double divisor, int nthreads, double *dos_l, double *dos_r,
-                 double *dos_bulk, int *iters)
+ * dos_r = -sum_{b in bot} Im GRR(N-nw+b, N-nw+b), dos_bulk = -sum_i Im GB(i,i),
+ * and with spin_mode != 0 the spin-resolved sums of spin_sums: left on GLL
+ * over top; right on GRR over top (spin_mode 1, surfstat) or over the bottom
+ * orbitals of the last layer (spin_mode 2, SurfaceDOSkk). */
+static int spectral_impl(int nw, int np, int nr, int ijmax, const int *irvec,
+                         const double *hr, int nk, const double *kpts, int nomega,
+                         const double *omega, double eta, int ntop, const int *top,
+                         int nbot, const int *bot, int itermax, double accuracy,
+                         double divisor, int nthreads, double *dos_l, double *dos_r,
+                         double *dos_bulk, int *iters, int spin_mode, double *spin_l,
+                         double *spin_r)
 {
     const int n = nw * np;
     if (nw <= 0 || np <= 0 || nk < 0 || nomega < 0 || itermax < 1) return -1;
+    if (spin_mode < 0 || spin_mode > 2 || (spin_mode && (!spin_l || !spin_r))) return -1;
     for (int i = 0; i < ntop; ++i) if (top[i] < 1 || top[i] > n) return -1;
     for (int i = 0; i < nbot; ++i) if (bot[i] < 1 || bot[i] > nw) return -1;
     const size_t nn = (size_t)n * n;
     cplx *hk = malloc(sizeof(cplx) * 2 * nn * (size_t)(nk > 0 ? nk : 1));
-    if (!hk) return -2;
+    cplx *sig = spin_mode ? malloc(sizeof(cplx) * 3 * nn) : NULL;
+    if (!hk || (spin_mode && !sig)) { free(hk); free(sig); return -2; }
+    if (spin_mode) spin_operators(nw, np, sig);
     int err = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -261,14 +309,15 @@ int wto_spectral(int nw, int np, int nr, int ijmax, const int *irvec,
                               (double *)(hk + 2 * nn * ik),
                               (double *)(hk + 2 * nn * ik + nn));
 
+    const size_t total_sz = (size_t)nk * nomega;
 #pragma omp parallel reduction(| : err)
     {
-        cplx *w = malloc(sizeof(cplx) * 12 * nn);
+        cplx *w = malloc(sizeof(cplx) * 13 * nn);
         int *ipiv = malloc(sizeof(int) * (size_t)n);
         if (!w || !ipiv) {
             err |= 2;
         } else {
-            cplx *gll = w + 9 * nn, *grr = w + 10 * nn, *gb = w + 11 * nn;
+            cplx *gll = w + 9 * nn, *grr = w + 10 * nn, *gb = w + 11 * nn, *ct = w + 12 * nn;
             const long total = (long)nk * nomega;
 #pragma omp for schedule(dynamic, 1)
             for (long t = 0; t < total; ++t) {
@@ -293,11 +342,50 @@ int wto_spectral(int nw, int np, int nr, int ijmax, const int *irvec,
                 dos_r[o] = dr;
                 dos_bulk[o] = db;
                 if (iters) iters[o] = it;
+                if (spin_mode) {
+                    double sl[3], sr[3];
+                    spin_sums(n, gll, sig, ct, top, ntop, 0, divisor, sl);
+                    if (spin_mode == 1)
+                        spin_sums(n, grr, sig, ct, top, ntop, 0, divisor, sr);
+                    else
+                        spin_sums(n, grr, sig, ct, bot, nbot, n - nw, divisor, sr);
+                    for (int c = 0; c < 3; ++c) {
+                        spin_l[o + c * total_sz] = sl[c];
+                        spin_r[o + c * total_sz] = sr[c];
+                    }
+                }
             }
         }
         free(w);
         free(ipiv);
     }
     free(hk);
+    free(sig);
     return err ? (err & 2 ? -2 : 1) : 0;
+}
+
+int wto_spectral(int nw, int np, int nr, int ijmax, const int *irvec,
+                 const double *hr, int nk, const double *kpts, int nomega,
+                 const double *omega, double eta, int ntop, const int *top,
+                 int nbot, const int *bot, int itermax, double accuracy,
+                 double divisor, int nthreads, double *dos_l, double *dos_r,
+                 double *dos_bulk, int *iters)
+{
+    return spectral_impl(nw, np, nr, ijmax, irvec, hr, nk, kpts, nomega, omega, eta, ntop, top,
+                         nbot, bot, itermax, accuracy, divisor, nthreads, dos_l, dos_r,
+                         dos_bulk, iters, 0, NULL, NULL);
+}
+
+int wto_spectral_spin(int nw, int np, int nr, int ijmax, const int *irvec,
+                      const double *hr, int nk, const double *kpts, int nomega,
+                      const double *omega, double eta, int ntop, const int *top,
+                      int nbot, const int *bot, int itermax, double accuracy,
+                      double divisor, int nthreads, double *dos_l, double *dos_r,
+                      double *dos_bulk, int *iters, int spin_mode, double *spin_l,
+                      double *spin_r)
+{
+    if (spin_mode < 1) return -1;
+    return spectral_impl(nw, np, nr, ijmax, irvec, hr, nk, kpts, nomega, omega, eta, ntop, top,
+                         nbot, bot, itermax, accuracy, divisor, nthreads, dos_l, dos_r,
+                         dos_bulk, iters, spin_mode, spin_l, spin_r);
 }

@@ -10,6 +10,8 @@
  *   wto_mat_mul            <- src/mat_mul.f90:3-27   (zgemm 'N','N')
  *   wto_surfgreen_1985     <- src/surfgreen.f90:7-137
  *   wto_spectral           <- src/surfstat.f90:123-160 / src/fermiarc.f90:206-248
+ *   wto_spectral_spin      <- src/surfstat.f90:102-115,163-192 /
+ *                             src/fermiarc.f90:85-101,250-328
  *
  * Conventions (all as in the reference): complex matrices are column-major,
  * interleaved (re, im) doubles, leading dimension = order; 1-based orbital
@@ -25,6 +27,10 @@
  *                                finite-slab brute force.
  *   wto_spectral              -- reference golden traces and the diagonal
  *                                sums of wto_surfgreen_1985.
+ *   wto_spectral_spin         -- the reference program's spindos.dat_l/_r
+ *                                (tests/golden/files_*), and spin-diagonal
+ *                                models where s_z must equal the up minus
+ *                                down spectral weight and s_x = s_y = 0.
  */
 #ifndef WTSURF_ORACLE_H
 #define WTSURF_ORACLE_H
@@ -63,6 +69,19 @@ int wto_spectral(int nw, int np, int nr, int ijmax, const int *irvec,
                  int nbot, const int *bot, int itermax, double accuracy,
                  double divisor, int nthreads, double *dos_l, double *dos_r,
                  double *dos_bulk, int *iters);
+
+/* wto_spectral plus the spin-resolved traces (src/surfstat.f90:102-115,
+ * 163-192; src/fermiarc.f90:85-101, 250-328): s_c = -sum Im (G sigma_c)(io,io)
+ * / divisor with G sigma_c formed as a full product.  Left: GLL over top.
+ * Right: GRR over top (spin_mode 1, surfstat) or over bot offset by N - nw
+ * (spin_mode 2, SurfaceDOSkk).  spin_l, spin_r are [3][nomega][nk]. */
+int wto_spectral_spin(int nw, int np, int nr, int ijmax, const int *irvec,
+                      const double *hr, int nk, const double *kpts, int nomega,
+                      const double *omega, double eta, int ntop, const int *top,
+                      int nbot, const int *bot, int itermax, double accuracy,
+                      double divisor, int nthreads, double *dos_l, double *dos_r,
+                      double *dos_bulk, int *iters, int spin_mode, double *spin_l,
+                      double *spin_r);
 
 #ifdef __cplusplus
 }

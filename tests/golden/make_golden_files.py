@@ -3,8 +3,8 @@
 Runs only where the reference tree exists: builds oracle/_ref/wt.x (the whole
 reference program, compiled unchanged from its sources by oracle/Makefile),
 runs it on small synthetic inputs and stores, per case, the inputs (wt.in,
-*_hr.dat) and the reference's dos.dat_{l,r,bulk} and arc.dat_{l,r,bulk}
-under tests/golden/files_<case>/.
+*_hr.dat) and the reference's dos.dat_{l,r,bulk}, spindos.dat_{l,r} (from
+surfstat) and arc.dat_{l,r,bulk} under tests/golden/files_<case>/.
 
     python tests/golden/make_golden_files.py
 """
@@ -25,7 +25,9 @@ from wannier_tools_amd import model as M  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 WTX = os.path.join(ROOT, "oracle", "_ref", "wt.x")
-OUTPUTS = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk"]
+OUTPUTS = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk",
+           # surfstat's spin-resolved traces (written whatever SOC is)
+           "spindos.dat_l", "spindos.dat_r"]
 
 WT_IN = """&TB_FILE
 Hrfile = '{hr}'
@@ -123,6 +125,13 @@ CASES = {
                        nomega=15, nk1=9, nk2=5,
                        lattice=["1.0 0.0 0.0", "0.0 1.0 0.0", "0.0 0.0 1.0"],
                        surface_atoms="SURFACE_ATOMS\n2\n1\n"),
+    # spinful, one atom x 5 orbitals x spin = 10, Np = 2 (Ndim = 20), on a
+    # lattice whose surface frame (Urot) is a general rotation
+    "soc_rot_ndim20": dict(mdl=lambda: M.synthetic_model(10, 1, 0.35, M.decay_l1, seed=12,
+                                                         soc_sigma=0.08, name="soc10rot"),
+                           soc=1, nprojs=[5], np_=2, eta=0.005, earc=0.15, efermi=0.02,
+                           nomega=11, nk1=7, nk2=5,
+                           lattice=["0.9 0.3 0.1", "-0.2 1.0 0.25", "0.1 -0.1 1.2"]),
 }
 
 

@@ -1,8 +1,8 @@
 """The Fortran host on the GPU against the reference program's own files.
 
 * wannier_tools_amd/bin/wt_surf -- the standalone driver (wt.in + *_hr.dat in,
-  dos.dat_* / arc.dat_* out) running surfstat and SurfaceDOSkk through
-  libwtsurf;
+  dos.dat_* / spindos.dat_* / arc.dat_* out) running surfstat and
+  SurfaceDOSkk through libwtsurf;
 * oracle/_ref/wt_gpu.x (where built) -- the reference program itself with
   the drop-in wtsurf_dropin.f90 linked ahead of its own surfstat /
   SurfaceDOSkk / surfgreen_1985 / ham_qlayer2qlayer.
@@ -19,8 +19,9 @@ from helpers import compare_tables
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["ndim16", "soc_ndim30"]
-FILES = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk"]
+CASES = ["ndim16", "soc_ndim30", "soc_rot_ndim20"]
+FILES = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk",
+         "spindos.dat_l", "spindos.dat_r"]
 WT_SURF = os.path.join(ROOT, "wannier_tools_amd", "bin", "wt_surf")
 WT_GPU = os.path.join(ROOT, "oracle", "_ref", "wt_gpu.x")
 

@@ -19,6 +19,8 @@ CASES = {
                    efermi=0.05, top_atoms=[1], bot_atoms=[1], nprojs=[8]),
     "soc_ndim30": dict(nw=10, np_=3, soc=1, nomega=15, nk1=9, nk2=5, eta_arc=0.004, e_arc=-0.2,
                        efermi=0.0, top_atoms=[2], bot_atoms=[1], nprojs=[3, 2]),
+    "soc_rot_ndim20": dict(nw=10, np_=2, soc=1, nomega=11, nk1=7, nk2=5, eta_arc=0.005,
+                           e_arc=0.15, efermi=0.02, top_atoms=[1], bot_atoms=[1], nprojs=[5]),
 }
 KPATH = [((0.0, 0.0), (0.5, 0.0)), ((0.5, 0.0), (0.5, 0.5))]
 
@@ -76,3 +78,47 @@ def test_oracle_reproduces_reference_arc_files(oracle_lib, name):
         ref = read_table(os.path.join(d, f"arc.dat_{s}"))[1]
         assert ref.shape[0] == nkx * nky
         assert np.allclose(v, ref[:, 2], rtol=0, atol=2e-8 + 2e-8 * np.abs(ref[:, 2]).max())
+
+
+LATTICE = {"ndim16": [[1.1, 0.0, 0.0], [0.3, 0.9, 0.0], [0.0, 0.1, 1.3]],
+           "soc_ndim30": [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]],
+           "soc_rot_ndim20": [[0.9, 0.3, 0.1], [-0.2, 1.0, 0.25], [0.1, -0.1, 1.2]]}
+
+
+def surface_rotation(r1, r2):
+    """Urot of src/readinput.f90:1816-1830 (rows: x along R1', z along
+    R1' x R2', y = z x x); rotate() (src/readinput.f90:3726-3737) is Urot @ s."""
+    r1, r2 = np.asarray(r1, float), np.asarray(r2, float)
+    u = np.zeros((3, 3))
+    u[0] = r1 / np.linalg.norm(r1)
+    u[2] = np.cross(r1, r2)
+    u[2] /= np.linalg.norm(u[2])
+    u[1] = np.cross(u[2], u[0])
+    u[1] /= np.linalg.norm(u[1])
+    return u
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_oracle_reproduces_reference_spindos_files(oracle_lib, name):
+    """surfstat's spindos.dat_l/_r: spin traces of GLL and of GRR, both over
+    the TopOrbitals indices (src/surfstat.f90:163-192), rotated into the
+    surface frame and printed 5(1X,E16.8) (src/surfstat.f90:270-302).  With
+    SOC = 0 the reference still pairs orbitals j and j + Num_wann/2."""
+    d, c, mdl = case_model(name)
+    irvec, hr = mdl.surface_cell()
+    om = M.omega_grid(-1.0 * M.EV2HARTREE, 1.0 * M.EV2HARTREE, c["nomega"])
+    eta = (1.0 * M.EV2HARTREE - (-1.0 * M.EV2HARTREE)) / c["nomega"] * 3.0
+    kp = M.kpath_slab(KPATH, c["nk1"])
+    top = orbitals(c["top_atoms"], c["nprojs"], c["soc"])
+    bot = orbitals(c["bot_atoms"], c["nprojs"], c["soc"])
+    *_, sl, sr = oracle_lib.spectral_spin(c["nw"], c["np_"], irvec, hr, kp, om, eta, top, bot,
+                                          spin_mode=1)
+    lat = LATTICE[name]
+    urot = surface_rotation(lat[0], lat[1])
+    for side, s in (("l", sl), ("r", sr)):
+        comments, ref, blanks = read_table(os.path.join(d, f"spindos.dat_{side}"))
+        assert len(comments) == 3 and ref.shape == (kp.shape[0] * om.size, 5)
+        # rows run k outer, omega inner; s is (3, nk, nomega)
+        got = np.einsum("ij,jkw->kwi", urot, s).reshape(-1, 3)
+        scale = np.abs(ref[:, 2:]).max()
+        assert np.all(np.abs(got - ref[:, 2:]) <= 2e-7 * np.abs(ref[:, 2:]) + 2e-9 * scale), side

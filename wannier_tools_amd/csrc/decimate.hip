@@ -384,6 +384,47 @@ __device__ __forceinline__ bool gj_inverse(unsigned char *sbuf, unsigned char *t
     return singular;
 }
 
+// Element (i, j) of the FP64 inverse that the last gj_inverse left in sbuf
+// (valid until the next gj_inverse on that buffer).
+template <int Q>
+__device__ __forceinline__ double2 gj_inverse_at(const unsigned char *sbuf, const Common<Q> &cm,
+                                                 int i, int j) {
+    constexpr int LD = 16 * Q + 1;
+    if (cm.sing) return make_double2(i == j ? 1.0 : 0.0, 0.0);
+    const double2(*S)[LD] = reinterpret_cast<const double2(*)[LD]>(sbuf);
+    return S[cm.pivstep[j]][cm.pivrow[i]];
+}
+
+// Spin-resolved traces -sum_{io in list} Im (G sigma_c)(io, io) / divisor,
+// c = x, y, z, for the spin operators of src/surfstat.f90:102-115 (and
+// src/fermiarc.f90:85-101): in every principal layer, orbitals 1..nw/2 are
+// spin up and nw/2+1..2(nw/2) their spin-down partners.  sigma has one entry
+// per column, so (G sigma)(io, io) is one element of G times 1, +-i or +-1:
+//   up io (partner p = io + nw/2):  x: G(io,p)    y: +i G(io,p)   z: +G(io,io)
+//   down io (p = io - nw/2):        x: G(io,p)    y: -i G(io,p)   z: -G(io,io)
+// and an odd nw's last orbital has no entry.  Each product is exact, so the
+// terms equal the reference's ZGEMM diagonal bit for bit.
+template <int Q>
+__device__ void spin_traces(const unsigned char *sbuf, const Common<Q> &cm, const int *list,
+                            int cnt, int offset, int nw, double divisor, double *out, size_t stride) {
+    const int nh = nw / 2;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int e = 0; e < cnt; ++e) {
+        const int io = offset + list[e] - 1;
+        const int l = io % nw;
+        if (l >= 2 * nh) continue;
+        const bool up = l < nh;
+        const double2 g = gj_inverse_at(sbuf, cm, io, up ? io + nh : io - nh);
+        const double2 d = gj_inverse_at(sbuf, cm, io, io);
+        sx = sx - g.y / divisor;
+        sy = sy - (up ? g.x : -g.x) / divisor;
+        sz = sz - (up ? d.y : -d.y) / divisor;
+    }
+    out[0] = sx;
+    out[stride] = sy;
+    out[2 * stride] = sz;
+}
+
 // ---------------------------------------------------------------------------
 // One complex 16x16x4 MFMA step on (re, im) fragments:
 //   accR += a.x b.x - a.y b.y,   accI += a.x b.y + a.y b.x
@@ -928,26 +969,35 @@ __device__ __forceinline__ void gj_round(const DecimateArgs &a, int slot, const 
                                                     full ? 2 : 1, full, nullptr);
         if (tid == 0) {
             if (sing) cm.s.status |= 1;
+            // output position: instance order, or (ik + nk * iw) on a grid
+            const int t = cm.s.t;
+            const int o = a.inst_k ? t : ik + a.nk * (t - ik * a.nomega);
             if (q == 0) {
                 double dl = 0.0;
                 for (int i = 0; i < a.ntop; ++i) dl = dl - cm.diag[a.top[i] - 1].y / a.divisor;
                 cm.s.dl = dl;
+                if (a.spin_mode)
+                    spin_traces(s64, cm, a.top, a.ntop, 0, a.nw, a.divisor, a.spin_out(o, 0),
+                                (size_t)a.ninst);
                 cm.s.phase = kFinal1;
             } else if (q == 1) {
                 double dr = 0.0;
                 for (int i = 0; i < a.nbot; ++i)
                     dr = dr - cm.diag[n - a.nw + a.bot[i] - 1].y / a.divisor;
                 cm.s.dr = dr;
+                // surfstat sums GRR's spin traces over the TopOrbitals indices
+                // (src/surfstat.f90:178-191); SurfaceDOSkk over the bottom
+                // orbitals of the last principal layer (src/fermiarc.f90:298-327)
+                if (a.spin_mode == 1)
+                    spin_traces(s64, cm, a.top, a.ntop, 0, a.nw, a.divisor,
+                                a.spin_out(o, 1), (size_t)a.ninst);
+                else if (a.spin_mode == 2)
+                    spin_traces(s64, cm, a.bot, a.nbot, n - a.nw, a.nw, a.divisor,
+                                a.spin_out(o, 1), (size_t)a.ninst);
                 cm.s.phase = kFinal2;
             } else {
                 double db = 0.0;
                 for (int i = 0; i < n; ++i) db = db - cm.diag[i].y / a.divisor;
-                const int t = cm.s.t;
-                int o = t;
-                if (!a.inst_k) {
-                    const int iw = t - ik * a.nomega;
-                    o = ik + a.nk * iw;
-                }
                 a.dos_l[o] = cm.s.dl;
                 a.dos_r[o] = cm.s.dr;
                 a.dos_bulk[o] = db;

@@ -76,6 +76,10 @@ struct wtsurf_ctx {
     // per-instance batch (wtsurf_spectral_batch)
     DevBuf<int> inst_k;
     DevBuf<double> inst_w, inst_eta;
+    // spin-resolved traces: mode for the next runs, the last run's outputs
+    int spin_mode = WTSURF_SPIN_OFF;
+    DevBuf<double> spin;
+    size_t spin_ninst = 0;  // instances of the last run that took spin traces (0: none)
     // decimation workspace
     DevBuf<unsigned char> work, state;
     DevBuf<unsigned> queue;
@@ -184,7 +188,7 @@ void wtsurf_ctx_destroy(wtsurf_ctx *c) {
     c->d_top.release(); c->d_bot.release();
     c->kpts.release(); c->omega.release(); c->phase.release(); c->hk.release();
     c->dos_l.release(); c->dos_r.release(); c->dos_b.release();
-    c->inst_k.release(); c->inst_w.release(); c->inst_eta.release();
+    c->inst_k.release(); c->inst_w.release(); c->inst_eta.release(); c->spin.release();
     c->iters.release(); c->status.release(); c->work.release(); c->state.release();
     c->queue.release();
     for (auto &e : c->ev)
@@ -343,6 +347,12 @@ static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const
     a.work = c->work.p; a.state = reinterpret_cast<SlotState *>(c->state.p); a.queue = c->queue.p;
     a.dos_l = c->dos_l.p; a.dos_r = c->dos_r.p; a.dos_bulk = c->dos_b.p;
     a.iters = c->iters.p; a.status = c->status.p; a.gfull = gfull;
+    c->spin_ninst = 0;
+    if (c->spin_mode != WTSURF_SPIN_OFF && !gfull) {
+        if (c->spin.ensure((size_t)6 * ninst)) return c->fail("out of device memory (spin traces)");
+        a.spin_mode = c->spin_mode;
+        a.spin = c->spin.p;
+    }
     RoundCtl ctl{};
     ctl.s[0] = c->stream;
     ctl.s[1] = c->aux;
@@ -356,6 +366,7 @@ static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const
         const hipError_t e = hipGetLastError();
         return e != hipSuccess ? c->hipfail(e, "decimation rounds") : c->fail("decimation rounds failed");
     }
+    if (a.spin) c->spin_ninst = (size_t)ninst;
     return 0;
 }
 
@@ -521,6 +532,25 @@ int wtsurf_spectral_batch(wtsurf_ctx *c, int nk, const double *kpts, int ninst, 
     WT_HIP(c, hipMemcpy(dos_bulk, c->dos_b.p, ni * sizeof(double), hipMemcpyDeviceToHost));
     if (iters) WT_HIP(c, hipMemcpy(iters, c->iters.p, ni * sizeof(int), hipMemcpyDeviceToHost));
     if (status) WT_HIP(c, hipMemcpy(status, c->status.p, ni * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int wtsurf_set_spin_traces(wtsurf_ctx *c, int mode) {
+    if (!c) return -1;
+    if (mode != WTSURF_SPIN_OFF && mode != WTSURF_SPIN_SURFSTAT && mode != WTSURF_SPIN_ARC)
+        return c->fail("unknown spin trace mode");
+    c->spin_mode = mode;
+    return 0;
+}
+
+int wtsurf_spin_download(wtsurf_ctx *c, double *spin_l, double *spin_r) {
+    if (!c) return -1;
+    if (c->spin_ninst == 0) return c->fail("the last run took no spin traces");
+    if (int r = use_device(c)) return r;
+    WT_HIP(c, hipStreamSynchronize(c->stream));
+    const size_t m = 3 * c->spin_ninst;
+    if (spin_l) WT_HIP(c, hipMemcpy(spin_l, c->spin.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    if (spin_r) WT_HIP(c, hipMemcpy(spin_r, c->spin.p + m, m * sizeof(double), hipMemcpyDeviceToHost));
     return 0;
 }
 

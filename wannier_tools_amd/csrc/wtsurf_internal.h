@@ -42,6 +42,14 @@ struct DecimateArgs {
     int *iters;             // [ninst]
     int *status;            // [ninst] bit0 singular solve, bit1 not converged
     double2 *gfull;         // [ninst][3][n*n] GLL, GRR, GB or null
+    // spin-resolved traces (src/surfstat.f90:163-192, src/fermiarc.f90:250-328):
+    // 0 none; 1 GRR's sums over the top list (surfstat); 2 over the bottom
+    // list in the last principal layer (SurfaceDOSkk)
+    int spin_mode;
+    double *spin;           // [2][3][ninst]: (left, right) x (sx, sy, sz), output order
+    __host__ __device__ double *spin_out(size_t o, int side) const {
+        return spin + (size_t)3 * side * ninst + o;
+    }
 };
 
 // Streams and host-visible progress counter of the round driver: two work

@@ -72,4 +72,7 @@ module para
    ! surface reciprocal vectors (1/Bohr)
    real(Dp), public, save :: Ka2(2)
    real(Dp), public, save :: Kb2(2)
+
+   ! rows: x along R1', z along R1' x R2', y = z x x (src/module.f90:839)
+   real(dp), public, save :: Urot(3, 3)
 end module para

@@ -242,7 +242,7 @@ contains
 
    subroutine read_wt_in(fname)
       character(len=*), intent(in) :: fname
-      real(dp) :: eta_arc, e_arc, lat(3, 3), r1(3), r2(3), s(3, 3), urot(3, 3), ra2(2), rb2(2)
+      real(dp) :: eta_arc, e_arc, lat(3, 3), r1(3), r2(3), s(3, 3), ra2(2), rb2(2)
       real(dp) :: vol, kstart(2), kend(2), kp(2, 64), ke(2, 64), k1(2), k2(2), t1, temp
       integer :: u, ios, i, j, ia, io, natoms, nk2lines, ntop_atoms, nbot_atoms, nproj_tot
       integer, allocatable :: nprojs(:), orbitals_start(:), topatoms(:), botatoms(:)
@@ -320,19 +320,20 @@ contains
       r1 = lat(:, 1)
       r2 = lat(:, 2)
       ! surface frame and reciprocal vectors (src/readinput.f90:1812-1855)
-      urot(1, :) = r1/sqrt(sum(r1**2))
-      urot(3, 1) = r1(2)*r2(3) - r1(3)*r2(2)
-      urot(3, 2) = r1(3)*r2(1) - r1(1)*r2(3)
-      urot(3, 3) = r1(1)*r2(2) - r1(2)*r2(1)
-      urot(3, :) = urot(3, :)/sqrt(sum(urot(3, :)**2))
-      urot(2, 1) = urot(3, 2)*urot(1, 3) - urot(3, 3)*urot(1, 2)
-      urot(2, 2) = urot(3, 3)*urot(1, 1) - urot(3, 1)*urot(1, 3)
-      urot(2, 3) = urot(3, 1)*urot(1, 2) - urot(3, 2)*urot(1, 1)
-      urot(2, :) = urot(2, :)/sqrt(sum(urot(2, :)**2))
-      ra2(1) = sum(urot(1, :)*r1)
-      ra2(2) = sum(urot(2, :)*r1)
-      rb2(1) = sum(urot(1, :)*r2)
-      rb2(2) = sum(urot(2, :)*r2)
+      Urot = 0d0
+      Urot(1, :) = r1/sqrt(sum(r1**2))
+      Urot(3, 1) = r1(2)*r2(3) - r1(3)*r2(2)
+      Urot(3, 2) = r1(3)*r2(1) - r1(1)*r2(3)
+      Urot(3, 3) = r1(1)*r2(2) - r1(2)*r2(1)
+      Urot(3, :) = Urot(3, :)/sqrt(sum(Urot(3, :)**2))
+      Urot(2, 1) = Urot(3, 2)*Urot(1, 3) - Urot(3, 3)*Urot(1, 2)
+      Urot(2, 2) = Urot(3, 3)*Urot(1, 1) - Urot(3, 1)*Urot(1, 3)
+      Urot(2, 3) = Urot(3, 1)*Urot(1, 2) - Urot(3, 2)*Urot(1, 1)
+      Urot(2, :) = Urot(2, :)/sqrt(sum(Urot(2, :)**2))
+      ra2(1) = sum(Urot(1, :)*r1)
+      ra2(2) = sum(Urot(2, :)*r1)
+      rb2(1) = sum(Urot(1, :)*r2)
+      rb2(2) = sum(Urot(2, :)*r2)
       vol = abs(ra2(1)*rb2(2) - rb2(1)*ra2(2))
       if (vol < 1e-6) stop ' >>> ERROR: cell_volume equal zero'
       Ka2(1) = 2d0*pi/vol*rb2(2)
@@ -472,6 +473,20 @@ contains
    end subroutine read_hr
 
 end module wt_surf_input
+
+
+!> A vector in the surface frame: s1 = Urot s0 (src/readinput.f90:3722-3737).
+!> In a WannierTools build the reference's own rotate is used instead.
+subroutine rotate(R1, R2)
+   use para, only: dp, Urot
+   implicit none
+   real(dp), intent(in) :: R1(3)
+   real(dp), intent(inout) :: R2(3)
+   integer :: i
+   do i = 1, 3
+      R2(i) = Urot(i, 1)*R1(1) + Urot(i, 2)*R1(2) + Urot(i, 3)*R1(3)
+   end do
+end subroutine rotate
 
 
 program wt_surf

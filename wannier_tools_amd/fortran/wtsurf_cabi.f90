@@ -11,6 +11,9 @@ module wtsurf_cabi
    integer(c_int), parameter :: WTSURF_FP32_REFINE = 1
    integer(c_int), parameter :: WTSURF_STATUS_SINGULAR = 1
    integer(c_int), parameter :: WTSURF_STATUS_NOT_CONVERGED = 2
+   integer(c_int), parameter :: WTSURF_SPIN_OFF = 0
+   integer(c_int), parameter :: WTSURF_SPIN_SURFSTAT = 1
+   integer(c_int), parameter :: WTSURF_SPIN_ARC = 2
 
    interface
       integer(c_int) function wtsurf_ctx_create(device, ctx) bind(C, name='wtsurf_ctx_create')
@@ -83,6 +86,19 @@ module wtsurf_cabi
          real(c_double), value :: eta, divisor
          real(c_double), intent(out) :: dos_l(*), dos_r(*), dos_bulk(*)
          integer(c_int), intent(out) :: iters(*), status(*)
+      end function
+
+      integer(c_int) function wtsurf_set_spin_traces(ctx, mode) bind(C, name='wtsurf_set_spin_traces')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: mode
+      end function
+
+      integer(c_int) function wtsurf_spin_download(ctx, spin_l, spin_r) &
+            bind(C, name='wtsurf_spin_download')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         real(c_double), intent(out) :: spin_l(*), spin_r(*)
       end function
 
       integer(c_int) function wtsurf_max_ndim() bind(C, name='wtsurf_max_ndim')

@@ -15,9 +15,9 @@
 ! the device in one call.  This file compiles against the reference's
 ! module.f90 (para, wmpi) or against para_subset.f90.
 !
-! Not covered here: the spin-resolved traces (spindos.dat_*, QPI and spin
-! texture outputs) and the phonon LO-TO branch; the drivers stop with a
-! message where the reference would need them.
+! Not covered here: QPI and spin texture outputs of SurfaceDOSkk and the
+! phonon LO-TO branch; the drivers stop with a message where the reference
+! would need them.
 module wtsurf_gpu
    use iso_c_binding
    use wtsurf_cabi
@@ -134,20 +134,23 @@ end subroutine surfgreen_1985
 
 !> Surface spectral function along KPATH_SLAB (src/surfstat.f90:1-605):
 !> eta = 3 (OmegaMax-OmegaMin)/OmegaNum, omega_j = OmegaMin + (j-1)(OmegaMax-OmegaMin)/OmegaNum,
-!> dos_l/dos_r/dos_bulk traces, log and *_only post-processing, and the
-!> files dos.dat_l, dos.dat_r, dos.dat_bulk in the reference's formats.
+!> dos_l/dos_r/dos_bulk traces, log and *_only post-processing, the spin-resolved
+!> traces (GRR's over the TopOrbitals indices, as the reference sums them), and
+!> the files dos.dat_l, dos.dat_r, dos.dat_bulk, spindos.dat_l, spindos.dat_r
+!> in the reference's formats.
 subroutine surfstat
    use wmpi
    use para
    use wtsurf_gpu
    use wtsurf_cabi
    implicit none
-   integer :: i, j, ikp, doslfile, dosrfile, dosbulkfile
-   real(dp) :: eta_broadening
+   integer :: i, j, ikp, doslfile, dosrfile, dosbulkfile, spindoslfile, spindosrfile
+   real(dp) :: eta_broadening, s0(3), s1(3)
    real(dp), allocatable :: omega(:), kpts(:, :)
    real(dp), allocatable :: dos_l(:, :), dos_r(:, :), dos_bulk(:, :)
    real(dp), allocatable :: dos_l_only(:, :), dos_r_only(:, :)
    real(dp), allocatable :: dos_l_raw(:, :), dos_r_raw(:, :), dos_bulk_raw(:, :)
+   real(dp), allocatable :: spin_l(:, :, :), spin_r(:, :, :)
    integer(c_int), allocatable :: iters(:, :), status(:, :)
 
    if (index(Particle, 'phonon') /= 0 .and. LOTO_correction) then
@@ -161,6 +164,7 @@ subroutine surfstat
    allocate(dos_l_only(knv2, omeganum), dos_r_only(knv2, omeganum))
    allocate(dos_l_raw(knv2, omeganum), dos_r_raw(knv2, omeganum), dos_bulk_raw(knv2, omeganum))
    allocate(iters(knv2, omeganum), status(knv2, omeganum))
+   allocate(spin_l(knv2, omeganum, 3), spin_r(knv2, omeganum, 3))
 
    eta_broadening = (omegamax - omegamin)/dble(omeganum)*3.0d0
    do i = 1, omeganum
@@ -172,9 +176,16 @@ subroutine surfstat
    end do
 
    call wtsurf_gpu_upload_model(.true.)
+   ! spin-resolved traces of src/surfstat.f90:163-192 from the same GLL, GRR
+   call wtsurf_gpu_check(wtsurf_set_spin_traces(wtsurf_gpu_ctx(), WTSURF_SPIN_SURFSTAT), &
+      'wtsurf_set_spin_traces')
    call wtsurf_gpu_check(wtsurf_spectral_grid(wtsurf_gpu_ctx(), int(knv2, c_int), kpts, &
       int(omeganum, c_int), omega, eta_broadening, 1.0d0, wtsurf_gpu_precision, &
       dos_l_raw, dos_r_raw, dos_bulk_raw, iters, status), 'wtsurf_spectral_grid')
+   call wtsurf_gpu_check(wtsurf_spin_download(wtsurf_gpu_ctx(), spin_l, spin_r), &
+      'wtsurf_spin_download')
+   call wtsurf_gpu_check(wtsurf_set_spin_traces(wtsurf_gpu_ctx(), WTSURF_SPIN_OFF), &
+      'wtsurf_set_spin_traces')
    if (any(iand(status, WTSURF_STATUS_SINGULAR) /= 0)) print *, 'something wrong with zgesv'
 
    ! src/surfstat.f90:237-247
@@ -198,29 +209,53 @@ subroutine surfstat
    dosrfile = outfileindex
    outfileindex = outfileindex + 1
    dosbulkfile = outfileindex
+   outfileindex = outfileindex + 1
+   spindoslfile = outfileindex
+   outfileindex = outfileindex + 1
+   spindosrfile = outfileindex
    open(unit=doslfile, file='dos.dat_l')
    open(unit=dosrfile, file='dos.dat_r')
    open(unit=dosbulkfile, file='dos.dat_bulk')
+   open(unit=spindoslfile, file='spindos.dat_l')
+   open(unit=spindosrfile, file='spindos.dat_r')
    write(doslfile, '("#", a12, 6a17)') ' k(1/Ang)', ' E(eV)', 'dos_l', 'dos_l_only'
    write(dosrfile, '("#", a12, 6a17)') ' k(1/Ang)', ' E(eV)', 'dos_r', 'dos_r_only'
    write(dosbulkfile, '("#", a12, 6a17)') ' k(1/Ang)', ' E(eV)', 'dos_bulk'
+   ! src/surfstat.f90:280-285
+   write(spindoslfile, '("#", a)') ' spin dos_l, the axis is rotated as '
+   write(spindoslfile, '("#", a)') " x is along R1', z is along R1'xR2', y is along z x y"
+   write(spindoslfile, '("#", a12, 6a17)') ' k(1/Ang)', ' E(eV)', 'sx', 'sy', 'sz'
+   write(spindosrfile, '("#", a)') ' spin dos_r, the axis is rotated as '
+   write(spindosrfile, '("#", a)') " x is along R1', z is along R1'xR2', y is along z x y"
+   write(spindosrfile, '("#", a12, 6a17)') ' k(1/Ang)', ' E(eV)', 'sx', 'sy', 'sz'
    do ikp = 1, knv2
       do j = 1, omeganum
          write(doslfile, 2002) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_l(ikp, j), dos_l_only(ikp, j)
          write(dosrfile, 2002) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_r(ikp, j), dos_r_only(ikp, j)
          write(dosbulkfile, 2003) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_bulk(ikp, j)
+         s0 = spin_l(ikp, j, :)
+         call rotate(s0, s1)
+         write(spindoslfile, 2001) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, s1
+         s0 = spin_r(ikp, j, :)
+         call rotate(s0, s1)
+         write(spindosrfile, 2001) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, s1
       end do
       write(doslfile, *)
       write(dosrfile, *)
       write(dosbulkfile, *)
+      write(spindoslfile, *)
+      write(spindosrfile, *)
    end do
    close(doslfile)
    close(dosrfile)
    close(dosbulkfile)
+   close(spindoslfile)
+   close(spindosrfile)
    write(stdout, *) 'ndim', ndim
    write(stdout, *) 'knv2,omeganum,eta_broadening', knv2, omeganum, eta_broadening/eV2Hartree
    write(stdout, *) 'calculate density of state successfully'
 
+2001 format(5(1X, E16.8))
 2002 format(4(1X, E16.8))
 2003 format(3(1X, E16.8))
 end subroutine surfstat

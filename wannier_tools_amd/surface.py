@@ -9,6 +9,9 @@ reference's Fortran routines:
 * ``SurfaceGF.spectral_grid``      -- the (k, omega) loops and traces of
   surfstat (src/surfstat.f90:123-160) and SurfaceDOSkk
   (src/fermiarc.f90:206-248)
+* ``SurfaceGF.set_spin_traces`` / ``spin_download`` -- the spin-resolved
+  traces of surfstat (src/surfstat.f90:163-192) and SurfaceDOSkk
+  (src/fermiarc.f90:250-328)
 
 There is no CPU fallback: constructing a ``SurfaceGF`` without the built
 library, or without a HIP device, raises.
@@ -59,8 +62,17 @@ SIGNATURES = {
     "wtsurf_sync": (ctypes.c_int, [ctypes.c_void_p]),
     "wtsurf_grid_timing": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp]),
     "wtsurf_grid_download": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _ip, _ip]),
+    "wtsurf_set_spin_traces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "wtsurf_spin_download": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "wtsurf_max_ndim": (ctypes.c_int, []),
 }
+
+# Spin-resolved trace modes (which orbitals GRR's spin sums run over)
+SPIN_OFF = 0
+SPIN_SURFSTAT = 1   # top orbital indices, as surfstat (src/surfstat.f90:178-191)
+SPIN_ARC = 2        # bottom orbitals of the last layer, as SurfaceDOSkk (src/fermiarc.f90:298-327)
+_SPIN = {"off": SPIN_OFF, "surfstat": SPIN_SURFSTAT, "arc": SPIN_ARC,
+         SPIN_OFF: SPIN_OFF, SPIN_SURFSTAT: SPIN_SURFSTAT, SPIN_ARC: SPIN_ARC}
 
 _lib: Optional[ctypes.CDLL] = None
 
@@ -108,6 +120,7 @@ class SurfaceGF:
         self.nw = 0
         self.np = 0
         self._grid = (0, 0)
+        self._last_shape = None  # output shape of the last grid or batch run
 
     def close(self) -> None:
         if getattr(self, "ctx", None) is not None and self.ctx.value:
@@ -225,7 +238,28 @@ class SurfaceGF:
                                                    _iptr(st)),
                     "wtsurf_spectral_batch")
         self._grid = (0, 0)
+        self._last_shape = (ki.size,)
         return out[0], out[1], out[2], it, st
+
+    def set_spin_traces(self, mode="surfstat") -> None:
+        """Take spin-resolved traces in the runs that follow: "surfstat",
+        "arc" or "off" (see include/wtsurf.h)."""
+        if mode not in _SPIN:
+            raise ValueError(f"unknown spin trace mode {mode!r}")
+        self._check(self.lib.wtsurf_set_spin_traces(self.ctx, _SPIN[mode]),
+                    "wtsurf_set_spin_traces")
+
+    def spin_download(self):
+        """(spin_l, spin_r) of the last run, each (3,) + that run's output
+        shape: sx, sy, sz."""
+        if self._last_shape is None:
+            raise WtsurfError("no run to download spin traces from")
+        shp = self._last_shape
+        sl = np.zeros(shp + (3,), dtype=np.float64, order="F")
+        sr = np.zeros_like(sl)
+        self._check(self.lib.wtsurf_spin_download(self.ctx, _dptr(sl), _dptr(sr)),
+                    "wtsurf_spin_download")
+        return np.moveaxis(sl, -1, 0), np.moveaxis(sr, -1, 0)
 
     # ---- resident split (inputs stay in device memory) -------------------
     def upload_grid(self, kpts, omegas) -> None:
@@ -241,6 +275,7 @@ class SurfaceGF:
         self._check(self.lib.wtsurf_grid_run_async(self.ctx, float(eta), float(divisor),
                                                    _PRECISION[precision]),
                     "wtsurf_grid_run_async")
+        self._last_shape = tuple(self._grid)
 
     def sync(self) -> None:
         self._check(self.lib.wtsurf_sync(self.ctx), "wtsurf_sync")
