@@ -140,6 +140,19 @@ int wtsurf_set_spin_traces(wtsurf_ctx *ctx, int mode);
  * dos_l.  Either pointer may be NULL. */
 int wtsurf_spin_download(wtsurf_ctx *ctx, double *spin_l, double *spin_r);
 
+/* The QPI autocorrelation of SurfaceDOSkk over a k-plane
+ * (src/fermiarc.f90:496-560): for every q = (iq1, iq2),
+ *   out(iq1, iq2) = sum_{ik1, ik2} sum_{f < nfield} a_f(ik1, ik2) a_f(ik1q, ik2q),
+ *   ik1q = mod(ik1 + iq1 - 2, m1) + 1,   ik2q = mod(ik2 + iq2 - 2, m2) + 1
+ * (1-based).  The reference takes m1 = nkx - 1, m2 = nky - 1 with nfield = 1
+ * (dos_*_only) for jdos, and m1 = Nk1 - 1, m2 = Nk2 - 1 with nfield = 4
+ * (dos_*_only, sx, sy, sz) for jsdos.  fields is [nfield][nky][nkx] and out
+ * [nky][nkx], ik1 fastest (the reference's (nkx, nky) arrays).  Requires
+ * 1 <= m1 <= nkx, 1 <= m2 <= nky, m1 <= 2048.  The sum is taken in a
+ * different order than the reference's loops. */
+int wtsurf_kplane_autocorrelation(wtsurf_ctx *ctx, int nkx, int nky, int m1, int m2,
+                                  int nfield, const double *fields, double *out);
+
 /* Largest Ndim the FP64 decimation kernels are built for. */
 int wtsurf_max_ndim(void);
 

@@ -67,6 +67,7 @@ class Oracle:
                                      ctypes.c_int, _ip, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _ip])
         lib.wto_spectral_spin.argtypes = lib.wto_spectral.argtypes + [ctypes.c_int, _dp, _dp]
+        lib.wto_kplane_autocorrelation.argtypes = [ctypes.c_int] * 5 + [_dp, _dp]
         self.lib = lib
 
     @staticmethod
@@ -172,6 +173,21 @@ class Oracle:
         if rc < 0:
             raise RuntimeError(f"wto_spectral_spin failed ({rc})")
         return dl, dr, db, its, np.moveaxis(sl, 2, 0), np.moveaxis(sr, 2, 0)
+
+
+    def kplane_autocorrelation(self, fields, m1, m2):
+        """fields (nfield, nkx, nky) indexed [f, ik1-1, ik2-1]; returns out
+        (nkx, nky) indexed [iq1-1, iq2-1]."""
+        f = np.asarray(fields, dtype=np.float64)
+        if f.ndim == 2:
+            f = f[None]
+        nfield, nkx, nky = f.shape
+        buf = np.ascontiguousarray(f.transpose(0, 2, 1))
+        out = np.zeros((nky, nkx), dtype=np.float64)
+        rc = self.lib.wto_kplane_autocorrelation(nkx, nky, m1, m2, nfield, _d(buf), _d(out))
+        if rc != 0:
+            raise RuntimeError(f"wto_kplane_autocorrelation failed ({rc})")
+        return out.T.copy()
 
 
 class RefHarness:

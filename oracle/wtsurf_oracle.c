@@ -389,3 +389,35 @@ int wto_spectral_spin(int nw, int np, int nr, int ijmax, const int *irvec,
                          nbot, bot, itermax, accuracy, divisor, nthreads, dos_l, dos_r,
                          dos_bulk, iters, spin_mode, spin_l, spin_r);
 }
+
+/* The QPI autocorrelation of SurfaceDOSkk (src/fermiarc.f90:496-560), as the
+ * reference loops it: for every q = (iq1, iq2) of the k-plane,
+ *   out(iq1, iq2) = sum_{ik2} sum_{ik1} sum_f a_f(ik1, ik2) a_f(ik1q, ik2q),
+ *   ik1q = mod(ik1 + iq1 - 2, m1) + 1,  ik2q = mod(ik2 + iq2 - 2, m2) + 1
+ * (1-based), the field terms of one (ik1, ik2) added in order before the
+ * running sum (jsdos, :545-549).  fields is [nfield][nky][nkx] (ik1 fastest,
+ * the reference's (nkx, nky) arrays), out is [nky][nkx]. */
+int wto_kplane_autocorrelation(int nkx, int nky, int m1, int m2, int nfield,
+                               const double *fields, double *out)
+{
+    if (nkx < 1 || nky < 1 || m1 < 1 || m2 < 1 || nfield < 1) return -1;
+    if (m1 > nkx || m2 > nky) return -1;
+    const size_t plane = (size_t)nkx * nky;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long iq = 0; iq < (long)plane; ++iq) {
+        const int iq1 = (int)(iq % nkx) + 1, iq2 = (int)(iq / nkx) + 1;
+        double acc = 0.0;
+        for (int ik2 = 1; ik2 <= nky; ++ik2)
+            for (int ik1 = 1; ik1 <= nkx; ++ik1) {
+                const int ik1q = (ik1 + iq1 - 2) % m1 + 1, ik2q = (ik2 + iq2 - 2) % m2 + 1;
+                const size_t p = (size_t)(ik1 - 1) + (size_t)nkx * (ik2 - 1);
+                const size_t pq = (size_t)(ik1q - 1) + (size_t)nkx * (ik2q - 1);
+                double term = fields[p] * fields[pq];
+                for (int f = 1; f < nfield; ++f)
+                    term += fields[f * plane + p] * fields[f * plane + pq];
+                acc += term;
+            }
+        out[iq] = acc;
+    }
+    return 0;
+}

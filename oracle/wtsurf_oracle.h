@@ -12,6 +12,7 @@
  *   wto_spectral           <- src/surfstat.f90:123-160 / src/fermiarc.f90:206-248
  *   wto_spectral_spin      <- src/surfstat.f90:102-115,163-192 /
  *                             src/fermiarc.f90:85-101,250-328
+ *   wto_kplane_autocorrelation <- src/fermiarc.f90:496-560 (QPI jdos / jsdos)
  *
  * Conventions (all as in the reference): complex matrices are column-major,
  * interleaved (re, im) doubles, leading dimension = order; 1-based orbital
@@ -31,6 +32,9 @@
  *                                (tests/golden/files_*), and spin-diagonal
  *                                models where s_z must equal the up minus
  *                                down spectral weight and s_x = s_y = 0.
+ *   wto_kplane_autocorrelation -- the reference program's arc.jdat_* /
+ *                                arc.jsdat_*, and an FFT cyclic correlation
+ *                                of the folded plane.
  */
 #ifndef WTSURF_ORACLE_H
 #define WTSURF_ORACLE_H
@@ -82,6 +86,14 @@ int wto_spectral_spin(int nw, int np, int nr, int ijmax, const int *irvec,
                       double divisor, int nthreads, double *dos_l, double *dos_r,
                       double *dos_bulk, int *iters, int spin_mode, double *spin_l,
                       double *spin_r);
+
+/* QPI autocorrelation over a k-plane (src/fermiarc.f90:496-560):
+ * out(q) = sum_{k} sum_f a_f(k) a_f((k + q) mod (m1, m2)) with the
+ * reference's 1-based index arithmetic; m1 = nkx - 1, m2 = nky - 1 for jdos,
+ * m1 = Nk1 - 1, m2 = Nk2 - 1 for jsdos.  fields [nfield][nky][nkx], out
+ * [nky][nkx], ik1 fastest. */
+int wto_kplane_autocorrelation(int nkx, int nky, int m1, int m2, int nfield,
+                               const double *fields, double *out);
 
 #ifdef __cplusplus
 }

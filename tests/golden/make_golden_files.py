@@ -3,8 +3,12 @@
 Runs only where the reference tree exists: builds oracle/_ref/wt.x (the whole
 reference program, compiled unchanged from its sources by oracle/Makefile),
 runs it on small synthetic inputs and stores, per case, the inputs (wt.in,
-*_hr.dat) and the reference's dos.dat_{l,r,bulk}, spindos.dat_{l,r} (from
-surfstat) and arc.dat_{l,r,bulk} under tests/golden/files_<case>/.
+*_hr.dat) and the reference's outputs under tests/golden/files_<case>/:
+dos.dat_{l,r,bulk} and spindos.dat_{l,r} of surfstat (SlabSS_calc) and
+arc.dat_{l,r,bulk} of SurfaceDOSkk (SlabArc_calc) at the top; the QPI run
+(SlabQPI_kplane_calc: arc.dat_*, arc.jdat_*, with SOC arc.jsdat_* and the
+spin texture spindos.dat_*) under qpi/ and the spin texture run
+(SlabSpintexture_calc) under spintexture/, each with the wt.in it ran.
 
     python tests/golden/make_golden_files.py
 """
@@ -33,8 +37,7 @@ WT_IN = """&TB_FILE
 Hrfile = '{hr}'
 /
 &CONTROL
-SlabSS_calc = T
-SlabArc_calc = T
+{control}
 /
 &SYSTEM
 SOC = {soc}
@@ -70,22 +73,33 @@ KPATH_SLAB
 G 0.0 0.0 X 0.5 0.0
 X 0.5 0.0 M 0.5 0.5
 KPLANE_SLAB
--0.5 -0.5
-1.0 0.0
-0.0 1.0
+{kplane}
 {surface_atoms}"""
+
+KPLANE_DEFAULT = "-0.5 -0.5\n1.0 0.0\n0.0 1.0"
+# driver flags of one reference run (src/main.f90:714-765) and the files it writes
+RUNS = {
+    "base": ("SlabSS_calc = T\nSlabArc_calc = T", None),
+    "qpi": ("SlabQPI_kplane_calc = T", ["arc.dat_l", "arc.dat_r", "arc.dat_bulk", "arc.jdat_l",
+                                        "arc.jdat_r", "arc.jsdat_l", "arc.jsdat_r",
+                                        "spindos.dat_l", "spindos.dat_r"]),
+    "spintexture": ("SlabSpintexture_calc = T", ["arc.dat_l", "arc.dat_r", "arc.dat_bulk",
+                                                 "spindos.dat_l", "spindos.dat_r"]),
+}
 
 PROJ = ["s", "pz", "px", "py", "dz2", "dxz", "dyz", "dxy", "dx2-y2"]
 
 
 def write_case(d: str, name: str, mdl: M.TBModel, soc: int, nprojs: list[int], np_: int,
                eta: float, earc: float, efermi: float, nomega: int, nk1: int, nk2: int,
-               lattice, surface_atoms: str = "") -> None:
+               lattice, surface_atoms: str = "", control: str = RUNS["base"][0],
+               kplane: str = KPLANE_DEFAULT) -> None:
     hrname = f"{name}_hr.dat"
     M.write_hr(os.path.join(d, hrname), mdl)
     atoms = "\n".join(f"C {0.1 * i:.3f} {0.05 * i:.3f} {0.2 * i:.3f}" for i in range(len(nprojs)))
     projlines = "\n".join("C " + " ".join(PROJ[:p]) for p in nprojs)
-    text = WT_IN.format(hr=hrname, soc=soc, efermi=efermi, eta=eta, earc=earc, nomega=nomega,
+    text = WT_IN.format(control=control, kplane=kplane,
+                        hr=hrname, soc=soc, efermi=efermi, eta=eta, earc=earc, nomega=nomega,
                         nk1=nk1, nk2=nk2, np=np_, a1=lattice[0], a2=lattice[1], a3=lattice[2],
                         natoms=len(nprojs), atoms=atoms, nprojs=" ".join(map(str, nprojs)),
                         projlines=projlines, surface_atoms=surface_atoms)
@@ -93,21 +107,25 @@ def write_case(d: str, name: str, mdl: M.TBModel, soc: int, nprojs: list[int], n
         f.write(text)
 
 
+def run_wtx(d: str, out: str) -> None:
+    env = dict(os.environ, OPENBLAS_NUM_THREADS="1")
+    r = subprocess.run([WTX], cwd=d, env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0 or not os.path.exists(os.path.join(d, out)):
+        raise RuntimeError(f"reference wt.x failed in {d}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+
+
 def run_reference(d: str) -> None:
     """One reference process per driver: SurfaceDOSkk accumulates dos_bulk into an
     array it never zeroes (src/fermiarc.f90:247 on the allocation of :77-82), so
     after surfstat has run in the same process arc.dat_bulk carries stale heap
     values.  A process of its own gives SurfaceDOSkk fresh (zero) memory."""
-    env = dict(os.environ, OPENBLAS_NUM_THREADS="1")
     text = open(os.path.join(d, "wt.in")).read()
     for flag, out in (("SlabSS_calc", "dos.dat_l"), ("SlabArc_calc", "arc.dat_l")):
         one = text.replace("SlabSS_calc = T", f"SlabSS_calc = {'T' if flag == 'SlabSS_calc' else 'F'}")
         one = one.replace("SlabArc_calc = T", f"SlabArc_calc = {'T' if flag == 'SlabArc_calc' else 'F'}")
         with open(os.path.join(d, "wt.in"), "w") as f:
             f.write(one)
-        r = subprocess.run([WTX], cwd=d, env=env, capture_output=True, text=True, timeout=600)
-        if r.returncode != 0 or not os.path.exists(os.path.join(d, out)):
-            raise RuntimeError(f"reference wt.x failed in {d}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
+        run_wtx(d, out)
     with open(os.path.join(d, "wt.in"), "w") as f:
         f.write(text)
 
@@ -116,7 +134,8 @@ CASES = {
     # 8 orbitals on one atom, Np = 2 (Ndim = 16), non-cubic lattice, E_fermi shift
     "ndim16": dict(mdl=M.model_ndim16, soc=0, nprojs=[8], np_=2, eta=0.002, earc=0.1,
                    efermi=0.05, nomega=21, nk1=11, nk2=7,
-                   lattice=["1.1 0.0 0.0", "0.3 0.9 0.0", "0.0 0.1 1.3"]),
+                   lattice=["1.1 0.0 0.0", "0.3 0.9 0.0", "0.0 0.1 1.3"],
+                   runs=["base", "qpi"]),
     # spinful: 2 atoms x (3 + 2) orbitals x spin = 10, Np = 3 (Ndim = 30),
     # SURFACE_ATOMS picks different atoms for the two surfaces
     "soc_ndim30": dict(mdl=lambda: M.synthetic_model(10, 1, 0.3, M.decay_l1, seed=11,
@@ -132,6 +151,16 @@ CASES = {
                            soc=1, nprojs=[5], np_=2, eta=0.005, earc=0.15, efermi=0.02,
                            nomega=11, nk1=7, nk2=5,
                            lattice=["0.9 0.3 0.1", "-0.2 1.0 0.25", "0.1 -0.1 1.2"]),
+    # the QPI and spin texture drivers on the model above: Nk1 even (nkx = Nk1 + 1,
+    # so jdos and jsdos wrap at different moduli) and a KPLANE_SLAB whose
+    # vectors QPI rounds outwards (src/fermiarc.f90:113-138)
+    "qpi_soc_ndim20": dict(mdl=lambda: M.synthetic_model(10, 1, 0.35, M.decay_l1, seed=12,
+                                                         soc_sigma=0.08, name="soc10rot"),
+                           soc=1, nprojs=[5], np_=2, eta=0.005, earc=0.15, efermi=0.02,
+                           nomega=11, nk1=8, nk2=5,
+                           lattice=["0.9 0.3 0.1", "-0.2 1.0 0.25", "0.1 -0.1 1.2"],
+                           kplane="-0.4 -0.3\n0.8 0.1\n-0.2 0.7",
+                           runs=["qpi", "spintexture"]),
 }
 
 
@@ -139,16 +168,24 @@ def main() -> None:
     subprocess.run(["make", "-s", "-j8", "-C", os.path.join(ROOT, "oracle"), "refexe"], check=True)
     for name, c in CASES.items():
         dst = os.path.join(OUT, f"files_{name}")
-        with tempfile.TemporaryDirectory() as d:
-            write_case(d, name, c["mdl"](), c["soc"], c["nprojs"], c["np_"], c["eta"], c["earc"],
-                       c["efermi"], c["nomega"], c["nk1"], c["nk2"], c["lattice"],
-                       c.get("surface_atoms", ""))
-            run_reference(d)
-            shutil.rmtree(dst, ignore_errors=True)
-            os.makedirs(dst)
-            for fn in ["wt.in", f"{name}_hr.dat"] + OUTPUTS:
-                shutil.copy(os.path.join(d, fn), os.path.join(dst, fn))
-        print(f"wrote {dst}")
+        shutil.rmtree(dst, ignore_errors=True)
+        for run in c.get("runs", ["base"]):
+            control, outputs = RUNS[run]
+            sub = dst if run == "base" else os.path.join(dst, run)
+            with tempfile.TemporaryDirectory() as d:
+                write_case(d, name, c["mdl"](), c["soc"], c["nprojs"], c["np_"], c["eta"],
+                           c["earc"], c["efermi"], c["nomega"], c["nk1"], c["nk2"], c["lattice"],
+                           c.get("surface_atoms", ""), control, c.get("kplane", KPLANE_DEFAULT))
+                if run == "base":
+                    run_reference(d)
+                    outputs = OUTPUTS
+                else:
+                    run_wtx(d, "arc.dat_l")
+                    outputs = [fn for fn in outputs if os.path.exists(os.path.join(d, fn))]
+                os.makedirs(sub, exist_ok=True)
+                for fn in ["wt.in", f"{name}_hr.dat"] + outputs:
+                    shutil.copy(os.path.join(d, fn), os.path.join(sub, fn))
+            print(f"wrote {sub}")
 
 
 if __name__ == "__main__":

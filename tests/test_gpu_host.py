@@ -26,16 +26,22 @@ WT_SURF = os.path.join(ROOT, "wannier_tools_amd", "bin", "wt_surf")
 WT_GPU = os.path.join(ROOT, "oracle", "_ref", "wt_gpu.x")
 
 
-def run_case(exe, name, tmp_path):
-    src = os.path.join(GOLDEN, f"files_{name}")
+# SurfaceDOSkk's QPI and spin texture drivers, each run in its own directory
+RUNS = [("ndim16", "qpi"), ("qpi_soc_ndim20", "qpi"), ("qpi_soc_ndim20", "spintexture")]
+
+
+def run_case(exe, name, tmp_path, run=None):
+    src = os.path.join(GOLDEN, f"files_{name}", run or "")
     for fn in ("wt.in", f"{name}_hr.dat"):
         shutil.copy(os.path.join(src, fn), tmp_path / fn)
-    print("run", os.path.basename(exe), name, flush=True)
+    print("run", os.path.basename(exe), name, run or "", flush=True)
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120,
                        env=dict(os.environ, OPENBLAS_NUM_THREADS="1"))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    files = FILES if run is None else sorted(
+        fn for fn in os.listdir(src) if fn not in ("wt.in", f"{name}_hr.dat"))
     worst = {}
-    for fn in FILES:
+    for fn in files:
         worst[fn] = compare_tables(str(tmp_path / fn), os.path.join(src, fn))
     print("  max abs diff of printed values:", worst, flush=True)
 
@@ -52,3 +58,15 @@ def test_wt_surf_reproduces_reference_files(name, tmp_path):
 @pytest.mark.parametrize("name", CASES)
 def test_reference_program_with_gpu_drop_in(name, tmp_path):
     run_case(WT_GPU, name, tmp_path)
+
+
+@pytest.mark.parametrize("name,run", RUNS)
+def test_wt_surf_qpi_and_spin_texture_files(name, run, tmp_path):
+    assert os.path.exists(WT_SURF), "wt_surf not built (run __graft_entry__.build())"
+    run_case(WT_SURF, name, tmp_path, run)
+
+
+@pytest.mark.skipif(not os.path.exists(WT_GPU), reason="reference program with the drop-in not built")
+@pytest.mark.parametrize("name,run", RUNS)
+def test_reference_program_with_gpu_drop_in_qpi_and_spin_texture(name, run, tmp_path):
+    run_case(WT_GPU, name, tmp_path, run)

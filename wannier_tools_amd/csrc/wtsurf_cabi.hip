@@ -554,6 +554,40 @@ int wtsurf_spin_download(wtsurf_ctx *c, double *spin_l, double *spin_r) {
     return 0;
 }
 
+int wtsurf_kplane_autocorrelation(wtsurf_ctx *c, int nkx, int nky, int m1, int m2, int nfield,
+                                  const double *fields, double *out) {
+    if (!c) return -1;
+    if (nkx < 1 || nky < 1 || nfield < 1 || !fields || !out) return c->fail("invalid arguments");
+    if (m1 < 1 || m2 < 1 || m1 > nkx || m2 > nky)
+        return c->fail("moduli must lie in 1..nkx and 1..nky");
+    if (m1 > kplane_corr_max_m()) return c->fail("k-plane too large for the autocorrelation kernel");
+    if (int r = use_device(c)) return r;
+    const size_t plane = (size_t)nkx * nky;
+    DevBuf<double> buf;
+    if (buf.ensure((size_t)nfield * plane + (size_t)(nfield + 1) * m1 * m2 + plane))
+        return c->fail("out of device memory (k-plane autocorrelation)");
+    KplaneCorrArgs k{};
+    k.fields = buf.p;
+    k.nkx = nkx; k.nky = nky; k.m1 = m1; k.m2 = m2; k.nfield = nfield;
+    k.folded = buf.p + (size_t)nfield * plane;
+    k.corr = k.folded + (size_t)nfield * m1 * m2;
+    k.out = k.corr + (size_t)m1 * m2;
+    int rc = 0;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(buf.p, fields, (size_t)nfield * plane * sizeof(double),
+                            hipMemcpyHostToDevice, c->stream)) != hipSuccess)
+        rc = c->hipfail(e, "upload");
+    else if (kplane_corr_launch(k, c->stream))
+        rc = c->fail("k-plane autocorrelation launch failed");
+    else if ((e = hipMemcpyAsync(out, k.out, plane * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream)) != hipSuccess ||
+             (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        rc = c->hipfail(e, "download");
+    if (rc) (void)hipStreamSynchronize(c->stream);
+    buf.release();
+    return rc;
+}
+
 int wtsurf_spectral_grid(wtsurf_ctx *c, int nk, const double *kpts, int nomega, const double *omega,
                          double eta, double divisor, int precision, double *dos_l, double *dos_r,
                          double *dos_bulk, int *iters, int *status) {

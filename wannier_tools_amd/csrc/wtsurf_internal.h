@@ -89,4 +89,16 @@ struct FourierArgs {
 };
 int fourier_launch(const FourierArgs &a, hipStream_t s);
 
+// QPI autocorrelation over a k-plane (src/fermiarc.f90:496-560), device buffers.
+constexpr int kKplaneMaxM = 2048;  // largest m1 (LDS rows of 3 m1 doubles)
+struct KplaneCorrArgs {
+    const double *fields;   // [nfield][nky][nkx]
+    int nkx, nky, m1, m2, nfield;
+    double *folded;         // [nfield][m2][m1] scratch
+    double *corr;           // [m2][m1] scratch
+    double *out;            // [nky][nkx]
+};
+int kplane_corr_max_m();
+int kplane_corr_launch(const KplaneCorrArgs &k, hipStream_t s);
+
 }  // namespace wtsurf

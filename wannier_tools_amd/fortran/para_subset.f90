@@ -25,10 +25,11 @@ module para
    integer, parameter :: stdout = 8
    integer, public, save :: outfileindex = 11932
 
-   ! unit constants (src/module.f90:711-728); eps9 is a real(4) literal there
+   ! unit constants (src/module.f90:711-728); eps6 and eps9 are real(4) literals there
    real(dp), parameter :: Angstrom2atomic = 1d0/0.529177210903d0
    real(dp), parameter :: eV2Hartree = 1d0/27.211385d0
    real(dp), parameter :: Pi = 3.14159265358979d0
+   real(dp), parameter :: eps6 = 1e-6
    real(dp), parameter :: eps9 = 1e-9
    complex(dp), parameter :: zi = (0.0d0, 1.0d0)
 
@@ -37,6 +38,7 @@ module para
    character(80) :: Particle = 'electron'
    logical :: SlabSS_calc = .false., SlabArc_calc = .false.
    logical :: SlabSpintexture_calc = .false., SlabQPI_kplane_calc = .false.
+   logical :: SlabQPI_calc = .false.
    logical :: LOTO_correction = .false.
    integer :: Soc = 0
    real(Dp) :: E_fermi = 0d0

@@ -2,13 +2,15 @@
 !
 ! Run in a WannierTools input directory: reads wt.in (the part the path
 ! uses) and the Wannier90 *_hr.dat it names, then runs SlabSS_calc (surfstat,
-! dos.dat_*) and/or SlabArc_calc (SurfaceDOSkk, arc.dat_*) exactly as the
-! reference's main program dispatches them (src/main.f90:714-733), and
-! reports the time of each in WT.out.
+! dos.dat_*, spindos.dat_*), SlabArc_calc, SlabQPI_calc / SlabQPI_kplane_calc
+! and SlabSpintexture_calc (SurfaceDOSkk: arc.dat_*, arc.jdat_*, arc.jsdat_*,
+! spindos.dat_*) exactly as the reference's main program dispatches them
+! (src/main.f90:714-765), and reports the time of each in WT.out.
 !
 ! Input subset (names, defaults and units as src/readinput.f90):
 !   &TB_FILE    Hrfile, Particle
-!   &CONTROL    SlabSS_calc, SlabArc_calc (other flags are accepted and ignored)
+!   &CONTROL    SlabSS_calc, SlabArc_calc, SlabQPI_calc, SlabQPI_kplane_calc,
+!               SlabSpintexture_calc (other flags are accepted and ignored)
 !   &SYSTEM     SOC, E_FERMI
 !   &PARAMETERS Eta_Arc, E_arc, OmegaNum, OmegaMin, OmegaMax, Nk1, Nk2, NP
 !   cards       LATTICE, ATOM_POSITIONS, PROJECTORS, SURFACE, KPATH_SLAB,
@@ -257,6 +259,8 @@ contains
       call get_logical('SlabArc_calc', SlabArc_calc)
       call get_logical('SlabSpintexture_calc', SlabSpintexture_calc)
       call get_logical('SlabQPI_kplane_calc', SlabQPI_kplane_calc)
+      call get_logical('SlabQPI_calc', SlabQPI_calc)
+      SlabQPI_kplane_calc = SlabQPI_kplane_calc .or. SlabQPI_calc  ! src/readinput.f90:171
       call get_logical('LOTO_correction', LOTO_correction)
       call get_int('SOC', Soc)
       call get_real('E_FERMI', E_fermi)
@@ -372,7 +376,7 @@ contains
          read(u, *) K2D_start
          read(u, *) K2D_vec1
          read(u, *) K2D_vec2
-      else if (SlabArc_calc) then
+      else if (SlabArc_calc .or. SlabSpintexture_calc) then
          stop 'ERROR: please set KPLANE_SLAB for arc or spintexture calculations'
       end if
 
@@ -515,6 +519,22 @@ program wt_surf
       call SurfaceDOSkk
       call system_clock(c1)
       write(stdout, '(a, f16.3, a)') ' Time cost for SlabArc is about ', dble(c1 - c0)/rate, ' s'
+   end if
+   if (SlabQPI_calc .or. SlabQPI_kplane_calc) then
+      write(stdout, *) ' '
+      write(stdout, *) 'Start to calculate the surface QPI'
+      call system_clock(c0, rate)
+      call SurfaceDOSkk
+      call system_clock(c1)
+      write(stdout, '(a, f16.3, a)') ' Time cost for SlabQPI is about ', dble(c1 - c0)/rate, ' s'
+   end if
+   if (SlabSpintexture_calc) then
+      write(stdout, *) ' '
+      write(stdout, *) 'Start to calculate the spin texture for surface'
+      call system_clock(c0, rate)
+      call SurfaceDOSkk
+      call system_clock(c1)
+      write(stdout, '(a, f16.3, a)') ' Time cost for SlabSpintexture is about ', dble(c1 - c0)/rate, ' s'
    end if
    call wtsurf_gpu_finalize()
    write(stdout, *) 'Congratulations! you finished the calculation.'

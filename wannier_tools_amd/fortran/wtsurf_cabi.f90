@@ -101,6 +101,15 @@ module wtsurf_cabi
          real(c_double), intent(out) :: spin_l(*), spin_r(*)
       end function
 
+      integer(c_int) function wtsurf_kplane_autocorrelation(ctx, nkx, nky, m1, m2, nfield, &
+            fields, out) bind(C, name='wtsurf_kplane_autocorrelation')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nkx, nky, m1, m2, nfield
+         real(c_double), intent(in) :: fields(*)
+         real(c_double), intent(out) :: out(*)
+      end function
+
       integer(c_int) function wtsurf_max_ndim() bind(C, name='wtsurf_max_ndim')
          import :: c_int
       end function

@@ -15,9 +15,9 @@
 ! the device in one call.  This file compiles against the reference's
 ! module.f90 (para, wmpi) or against para_subset.f90.
 !
-! Not covered here: QPI and spin texture outputs of SurfaceDOSkk and the
-! phonon LO-TO branch; the drivers stop with a message where the reference
-! would need them.
+! Not covered here: the phonon LO-TO branch (the drivers stop with a message
+! where the reference would need it) and the gnuplot scripts the reference
+! writes beside its data files.
 module wtsurf_gpu
    use iso_c_binding
    use wtsurf_cabi
@@ -261,27 +261,38 @@ subroutine surfstat
 end subroutine surfstat
 
 
-!> Surface spectral function on KPLANE_SLAB at iso_energy (src/fermiarc.f90:1-494,
-!> the arc part): k-plane with Nk rounded up to odd, eta = Fermi_broadening,
-!> traces divided by pi, the max/10 thresholds and arc.dat_l, arc.dat_r,
-!> arc.dat_bulk in the reference's formats.
+!> Surface spectral function on KPLANE_SLAB at iso_energy (src/fermiarc.f90:1-494)
+!> with its spin texture and QPI (src/fermiarc.f90:85-138, 250-586): k-plane with
+!> Nk rounded up to odd (and, for QPI, the KPLANE_SLAB vectors rounded outwards),
+!> eta = Fermi_broadening, traces divided by pi, the max/10 thresholds (which
+!> also reset the spins), arc.dat_l, arc.dat_r, arc.dat_bulk; with SOC > 0 and
+!> spin texture or QPI the spin traces (GRR's over the bottom orbitals) and
+!> spindos.dat_l, spindos.dat_r; with QPI the jdos / jsdos autocorrelations on
+!> the GPU and arc.jdat_l, arc.jdat_r (and with SOC > 0 arc.jsdat_l,
+!> arc.jsdat_r), all in the reference's formats.
 subroutine SurfaceDOSkk
    use wmpi
    use para
    use wtsurf_gpu
    use wtsurf_cabi
    implicit none
-   integer :: i, j, ikp, nkx, nky, arclfile, arcrfile, arcbulkfile
-   real(dp) :: omega(1), eta_broadening, dos_l_max, dos_r_max
+   integer :: i, j, ikp, nkx, nky, ik1, ik2, arclfile, arcrfile, arcbulkfile, spinfile, jfile
+   real(dp) :: omega(1), eta_broadening, dos_l_max, dos_r_max, s0(3), s1(3)
+   real(dp) :: K2D_vec_a(2), K2D_vec_b(2)
+   logical :: spin, qpi
    integer, allocatable :: ik12(:, :)
    real(dp), allocatable :: k12(:, :), k12_shape(:, :)
-   real(dp), allocatable :: dl(:), dr(:), db(:)
+   real(dp), allocatable :: dl(:), dr(:), db(:), sl(:, :), sr(:, :)
    real(dp), allocatable :: dos_l(:, :), dos_r(:, :), dos_bulk(:, :)
    real(dp), allocatable :: dos_l_only(:, :), dos_r_only(:, :)
+   real(dp), allocatable :: spin_l(:, :, :), spin_r(:, :, :)
+   real(dp), allocatable :: fields(:, :, :), corr(:, :), jdos(:, :), jsdos(:, :)
    integer(c_int), allocatable :: iters(:), status(:)
 
-   if (SlabQPI_kplane_calc .or. (Soc > 0 .and. SlabSpintexture_calc)) then
-      write(*, '(a)') ' >>> ERROR: QPI and spin texture are not part of the GPU SurfaceDOSkk'
+   qpi = SlabQPI_kplane_calc
+   spin = Soc > 0 .and. (SlabSpintexture_calc .or. SlabQPI_kplane_calc)
+   if (spin .and. index(Particle, 'phonon') /= 0) then
+      write(*, '(a)') ' >>> ERROR: no spin texture for a phonon system'
       stop 1
    end if
    if (index(Particle, 'phonon') /= 0 .and. LOTO_correction) then
@@ -294,6 +305,28 @@ subroutine SurfaceDOSkk
    nky = Nk2
    if (mod(Nk1, 2) == 0) nkx = Nk1 + 1
    if (mod(Nk2, 2) == 0) nky = Nk2 + 1
+   ! QPI needs whole reciprocal vectors: each component rounded away from zero
+   ! (src/fermiarc.f90:113-138)
+   if (qpi) then
+      do i = 1, 2
+         if (K2D_vec1(i) > 0) then
+            K2D_vec_a(i) = ceiling(K2D_vec1(i))
+         else
+            K2D_vec_a(i) = floor(K2D_vec1(i))
+         end if
+         if (K2D_vec2(i) > 0) then
+            K2D_vec_b(i) = ceiling(K2D_vec2(i))
+         else
+            K2D_vec_b(i) = floor(K2D_vec2(i))
+         end if
+      end do
+      write(stdout, '(a)') 'WARNING : KPLANE_SLAB vectors rounded to whole reciprocal vectors for QPI'
+      write(stdout, '((a, 2f8.4))') 'The first modified vector in QPI: ', K2D_vec_a
+      write(stdout, '((a, 2f8.4))') 'The second modified vector in QPI: ', K2D_vec_b
+   else
+      K2D_vec_a = K2D_vec1
+      K2D_vec_b = K2D_vec2
+   end if
    allocate(ik12(2, nkx*nky), k12(2, nkx*nky), k12_shape(2, nkx*nky))
    allocate(dl(nkx*nky), dr(nkx*nky), db(nkx*nky), iters(nkx*nky), status(nkx*nky))
    allocate(dos_l(nkx, nky), dos_r(nkx, nky), dos_bulk(nkx, nky))
@@ -304,7 +337,7 @@ subroutine SurfaceDOSkk
          ikp = ikp + 1
          ik12(1, ikp) = i
          ik12(2, ikp) = j
-         k12(:, ikp) = K2D_start + (i - 1)*K2D_vec1/dble(nkx - 1) + (j - 1)*K2D_vec2/dble(nky - 1)
+         k12(:, ikp) = K2D_start + (i - 1)*K2D_vec_a/dble(nkx - 1) + (j - 1)*K2D_vec_b/dble(nky - 1)
          k12_shape(:, ikp) = k12(1, ikp)*Ka2 + k12(2, ikp)*Kb2
       end do
    end do
@@ -314,24 +347,49 @@ subroutine SurfaceDOSkk
    if (index(Particle, 'phonon') /= 0) omega = omega*omega
 
    call wtsurf_gpu_upload_model(.true.)
+   if (spin) then
+      allocate(sl(nkx*nky, 3), sr(nkx*nky, 3), spin_l(nkx, nky, 3), spin_r(nkx, nky, 3))
+      call wtsurf_gpu_check(wtsurf_set_spin_traces(wtsurf_gpu_ctx(), WTSURF_SPIN_ARC), &
+         'wtsurf_set_spin_traces')
+   end if
    call wtsurf_gpu_check(wtsurf_spectral_grid(wtsurf_gpu_ctx(), int(nkx*nky, c_int), k12, &
       1_c_int, omega, eta_broadening, pi, wtsurf_gpu_precision, dl, dr, db, iters, status), &
       'wtsurf_spectral_grid')
+   if (spin) then
+      call wtsurf_gpu_check(wtsurf_spin_download(wtsurf_gpu_ctx(), sl, sr), 'wtsurf_spin_download')
+      call wtsurf_gpu_check(wtsurf_set_spin_traces(wtsurf_gpu_ctx(), WTSURF_SPIN_OFF), &
+         'wtsurf_set_spin_traces')
+   end if
    if (any(iand(status, WTSURF_STATUS_SINGULAR) /= 0)) print *, 'something wrong with zgesv'
    do ikp = 1, nkx*nky
-      dos_l(ik12(1, ikp), ik12(2, ikp)) = dl(ikp)
-      dos_r(ik12(1, ikp), ik12(2, ikp)) = dr(ikp)
-      dos_bulk(ik12(1, ikp), ik12(2, ikp)) = db(ikp)
+      ik1 = ik12(1, ikp)
+      ik2 = ik12(2, ikp)
+      dos_l(ik1, ik2) = dl(ikp)
+      dos_r(ik1, ik2) = dr(ikp)
+      dos_bulk(ik1, ik2) = db(ikp)
+      if (spin) then
+         spin_l(ik1, ik2, :) = sl(ikp, :)
+         spin_r(ik1, ik2, :) = sr(ikp, :)
+      end if
    end do
 
-   ! src/fermiarc.f90:372-397
+   ! src/fermiarc.f90:372-397: below a tenth of the maximum, dos_*_only and the
+   ! spins are reset to eps9
    dos_l_max = maxval(dos_l)
    dos_r_max = maxval(dos_r)
    do ikp = 1, nkx*nky
-      dos_l_only(ik12(1, ikp), ik12(2, ikp)) = dos_l(ik12(1, ikp), ik12(2, ikp))
-      if (dos_l_only(ik12(1, ikp), ik12(2, ikp)) < dos_l_max/10d0) dos_l_only(ik12(1, ikp), ik12(2, ikp)) = eps9
-      dos_r_only(ik12(1, ikp), ik12(2, ikp)) = dos_r(ik12(1, ikp), ik12(2, ikp))
-      if (dos_r_only(ik12(1, ikp), ik12(2, ikp)) < dos_r_max/10d0) dos_r_only(ik12(1, ikp), ik12(2, ikp)) = eps9
+      ik1 = ik12(1, ikp)
+      ik2 = ik12(2, ikp)
+      dos_l_only(ik1, ik2) = dos_l(ik1, ik2)
+      if (dos_l_only(ik1, ik2) < dos_l_max/10d0) then
+         dos_l_only(ik1, ik2) = eps9
+         if (spin) spin_l(ik1, ik2, :) = eps9
+      end if
+      dos_r_only(ik1, ik2) = dos_r(ik1, ik2)
+      if (dos_r_only(ik1, ik2) < dos_r_max/10d0) then
+         dos_r_only(ik1, ik2) = eps9
+         if (spin) spin_r(ik1, ik2, :) = eps9
+      end if
    end do
 
    outfileindex = outfileindex + 100
@@ -373,4 +431,119 @@ subroutine SurfaceDOSkk
    close(arclfile)
    close(arcrfile)
    close(arcbulkfile)
+
+   ! spin texture: s / dos where dos_*_only > eps6, in the surface frame
+   ! (src/fermiarc.f90:455-488)
+   if (spin) then
+      do i = 1, 2
+         outfileindex = outfileindex + 1
+         spinfile = outfileindex
+         if (i == 1) open(unit=spinfile, file='spindos.dat_l')
+         if (i == 2) open(unit=spinfile, file='spindos.dat_r')
+         write(spinfile, '(a)') '# The coordinates of sx,sy,sz are redefined according to the SURFACE card'
+         write(spinfile, '(a)') "# x axis is parallel to R1'"
+         write(spinfile, '(a)') "# z axis is parallel to R1'xR2'"
+         write(spinfile, '(a)') "# y axis is parallel to z x x"
+         write(spinfile, '(30a16)') '#kx', 'ky', 'sx', 'sy', 'sz'
+         do ikp = 1, nkx*nky
+            ik1 = ik12(1, ikp)
+            ik2 = ik12(2, ikp)
+            if (i == 1) then
+               if (.not. dos_l_only(ik1, ik2) > eps6) cycle
+               s0 = spin_l(ik1, ik2, :)/dos_l(ik1, ik2)
+            else
+               if (.not. dos_r_only(ik1, ik2) > eps6) cycle
+               s0 = spin_r(ik1, ik2, :)/dos_r(ik1, ik2)
+            end if
+            call rotate(s0, s1)
+            write(spinfile, '(30f16.8)') k12_shape(:, ikp)*Angstrom2atomic, s1
+         end do
+         close(spinfile)
+      end do
+   end if
+   write(stdout, *) 'Ndim: ', ndim
+   write(stdout, *) 'Nk1,Nk2,eta_broadening: ', nkx, nky, eta_broadening
+   write(stdout, *) 'Calculated surface density of state successfully'
+
+   if (.not. qpi) return
+   ! QPI (src/fermiarc.f90:496-586): jdos over dos_*_only with moduli nkx-1,
+   ! nky-1, normalised by nkx nky; with SOC jsdos over (dos_*_only, sx, sy, sz)
+   ! with moduli Nk1-1, Nk2-1, not normalised
+   allocate(corr(nkx, nky), jdos(nkx*nky, 2))
+   if (Soc > 0) allocate(fields(nkx, nky, 4), jsdos(nkx*nky, 2))
+   do i = 1, 2
+      if (i == 1) then
+         call wtsurf_gpu_check(wtsurf_kplane_autocorrelation(wtsurf_gpu_ctx(), int(nkx, c_int), &
+            int(nky, c_int), int(nkx - 1, c_int), int(nky - 1, c_int), 1_c_int, dos_l_only, corr), &
+            'wtsurf_kplane_autocorrelation')
+      else
+         call wtsurf_gpu_check(wtsurf_kplane_autocorrelation(wtsurf_gpu_ctx(), int(nkx, c_int), &
+            int(nky, c_int), int(nkx - 1, c_int), int(nky - 1, c_int), 1_c_int, dos_r_only, corr), &
+            'wtsurf_kplane_autocorrelation')
+      end if
+      do ikp = 1, nkx*nky
+         jdos(ikp, i) = corr(ik12(1, ikp), ik12(2, ikp))/nkx/nky
+      end do
+      if (Soc > 0) then
+         if (i == 1) then
+            fields(:, :, 1) = dos_l_only
+            fields(:, :, 2:4) = spin_l
+         else
+            fields(:, :, 1) = dos_r_only
+            fields(:, :, 2:4) = spin_r
+         end if
+         call wtsurf_gpu_check(wtsurf_kplane_autocorrelation(wtsurf_gpu_ctx(), int(nkx, c_int), &
+            int(nky, c_int), int(Nk1 - 1, c_int), int(Nk2 - 1, c_int), 4_c_int, fields, corr), &
+            'wtsurf_kplane_autocorrelation')
+         do ikp = 1, nkx*nky
+            jsdos(ikp, i) = corr(ik12(1, ikp), ik12(2, ikp))
+         end do
+      end if
+   end do
+   write(stdout, *) 'The calculation of joint density of state was done.'
+   do i = 1, 2
+      outfileindex = outfileindex + 1
+      jfile = outfileindex
+      if (i == 1) then
+         open(unit=jfile, file='arc.jdat_l')
+         write(jfile, '(a)') '# Surface joint density of states of bottom surface'
+      else
+         open(unit=jfile, file='arc.jdat_r')
+         write(jfile, '(a)') '# Surface joint density of states of top surface'
+      end if
+      write(jfile, '(a)') '# The coordinate of k is redefined according to the SURFACE card'
+      write(jfile, '(a)') "# x axis is parallel to R1'"
+      write(jfile, '(a)') "# z axis is parallel to R1'xR2'"
+      write(jfile, '(a)') "# y axis is parallel to z x x"
+      write(jfile, '(30a16)') '#kx', 'ky', 'log(dos)'
+      do ikp = 1, nkx*nky
+         write(jfile, '(30f16.8)') k12_shape(:, ikp)*Angstrom2atomic, log(abs(jdos(ikp, i)))
+         if (mod(ikp, nky) == 0) write(jfile, *) ' '
+      end do
+      close(jfile)
+   end do
+   if (Soc > 0) then
+      do i = 1, 2
+         outfileindex = outfileindex + 1
+         jfile = outfileindex
+         if (i == 1) then
+            open(unit=jfile, file='arc.jsdat_l')
+            write(jfile, '(a)') '# Surface joint density of states(in consideration of spin contribution) of bottom surface'
+         else
+            open(unit=jfile, file='arc.jsdat_r')
+            write(jfile, '(a)') '# Surface joint density of states(in consideration of spin contribution) of top surface'
+         end if
+         write(jfile, '(a)') '# The coordinate of k is redefined according to the SURFACE card'
+         write(jfile, '(a)') "# x axis is parallel to R1'"
+         write(jfile, '(a)') "# z axis is parallel to R1'xR2'"
+         write(jfile, '(a)') "# y axis is parallel to z x x"
+         write(jfile, '(30a16)') '#kx', 'ky', 'log(dos)'
+         do ikp = 1, nkx*nky
+            write(jfile, '(30f16.8)') k12_shape(:, ikp)*Angstrom2atomic, log(abs(jsdos(ikp, i)))
+            if (mod(ikp, nky) == 0) write(jfile, *) ' '
+         end do
+         close(jfile)
+      end do
+   end if
+   write(stdout, *) 'calculate joint density of state successfully'
 end subroutine SurfaceDOSkk

@@ -12,6 +12,8 @@ reference's Fortran routines:
 * ``SurfaceGF.set_spin_traces`` / ``spin_download`` -- the spin-resolved
   traces of surfstat (src/surfstat.f90:163-192) and SurfaceDOSkk
   (src/fermiarc.f90:250-328)
+* ``SurfaceGF.kplane_autocorrelation`` -- the QPI jdos / jsdos sums of
+  SurfaceDOSkk (src/fermiarc.f90:496-560)
 
 There is no CPU fallback: constructing a ``SurfaceGF`` without the built
 library, or without a HIP device, raises.
@@ -64,6 +66,8 @@ SIGNATURES = {
     "wtsurf_grid_download": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, _dp, _ip, _ip]),
     "wtsurf_set_spin_traces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "wtsurf_spin_download": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
+    "wtsurf_kplane_autocorrelation": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5
+                                      + [_dp, _dp]),
     "wtsurf_max_ndim": (ctypes.c_int, []),
 }
 
@@ -260,6 +264,21 @@ class SurfaceGF:
         self._check(self.lib.wtsurf_spin_download(self.ctx, _dptr(sl), _dptr(sr)),
                     "wtsurf_spin_download")
         return np.moveaxis(sl, -1, 0), np.moveaxis(sr, -1, 0)
+
+    def kplane_autocorrelation(self, fields, m1: int, m2: int) -> np.ndarray:
+        """QPI autocorrelation (see include/wtsurf.h): fields (nfield, nkx, nky)
+        or (nkx, nky) indexed [f, ik1-1, ik2-1]; returns (nkx, nky) indexed
+        [iq1-1, iq2-1]."""
+        f = np.asarray(fields, dtype=np.float64)
+        if f.ndim == 2:
+            f = f[None]
+        nfield, nkx, nky = f.shape
+        buf = np.ascontiguousarray(f.transpose(0, 2, 1))
+        out = np.zeros((nky, nkx), dtype=np.float64)
+        self._check(self.lib.wtsurf_kplane_autocorrelation(self.ctx, nkx, nky, int(m1), int(m2),
+                                                           nfield, _dptr(buf), _dptr(out)),
+                    "wtsurf_kplane_autocorrelation")
+        return out.T.copy()
 
     # ---- resident split (inputs stay in device memory) -------------------
     def upload_grid(self, kpts, omegas) -> None:
