@@ -153,6 +153,16 @@ int wtsurf_spin_download(wtsurf_ctx *ctx, double *spin_l, double *spin_r);
 int wtsurf_kplane_autocorrelation(wtsurf_ctx *ctx, int nkx, int nky, int m1, int m2,
                                   int nfield, const double *fields, double *out);
 
+/* The k-path joint density of states of surfstat_jdos
+ * (src/surfstat.f90:891-907), each field and energy on its own: with
+ * h = (nk - 1)/2 and iq1 = iq - h,
+ *   out_f(iq, j) = sum_{ik1 = max(1, 1 - iq1)}^{min(2h + 1, 2h + 1 - iq1)} a_f(ik1, j) a_f(ik1 + iq1, j)
+ * (1-based; the reference's imin1 / imax1).  fields and out are
+ * [nfield][nomega][nk], k fastest (the reference's dos_l(knv2, omeganum)).
+ * Each sum is taken in the reference's order. */
+int wtsurf_kpath_jdos(wtsurf_ctx *ctx, int nk, int nomega, int nfield,
+                      const double *fields, double *out);
+
 /* Largest Ndim the FP64 decimation kernels are built for. */
 int wtsurf_max_ndim(void);
 

@@ -68,6 +68,7 @@ class Oracle:
                                      ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, _ip])
         lib.wto_spectral_spin.argtypes = lib.wto_spectral.argtypes + [ctypes.c_int, _dp, _dp]
         lib.wto_kplane_autocorrelation.argtypes = [ctypes.c_int] * 5 + [_dp, _dp]
+        lib.wto_kpath_jdos.argtypes = [ctypes.c_int] * 3 + [_dp, _dp]
         self.lib = lib
 
     @staticmethod
@@ -188,6 +189,22 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"wto_kplane_autocorrelation failed ({rc})")
         return out.T.copy()
+
+    def kpath_jdos(self, fields):
+        """fields (nfield, nk, nomega) or (nk, nomega) indexed [f, ik-1, j-1]
+        (the reference's dos_l(knv2, omeganum)); returns the same shape."""
+        f = np.asarray(fields, dtype=np.float64)
+        single = f.ndim == 2
+        if single:
+            f = f[None]
+        nfield, nk, nom = f.shape
+        buf = np.ascontiguousarray(f.transpose(0, 2, 1))
+        out = np.zeros_like(buf)
+        rc = self.lib.wto_kpath_jdos(nk, nom, nfield, _d(buf), _d(out))
+        if rc != 0:
+            raise RuntimeError(f"wto_kpath_jdos failed ({rc})")
+        out = out.transpose(0, 2, 1).copy()
+        return out[0] if single else out
 
 
 class RefHarness:

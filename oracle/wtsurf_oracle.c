@@ -421,3 +421,31 @@ int wto_kplane_autocorrelation(int nkx, int nky, int m1, int m2, int nfield,
     }
     return 0;
 }
+
+/* The k-path joint density of states of surfstat_jdos (src/surfstat.f90:891-907),
+ * as the reference loops it, for each field a and energy j separately:
+ *   out(iq, j) = sum_{ik1 = imin1}^{imax1} a(ik1, j) a(ik1 + iq1, j),
+ *   Nk_half = (nk - 1)/2, iq1 = iq - Nk_half,
+ *   imin1 = max(-Nk_half - iq1, -Nk_half) + Nk_half + 1,
+ *   imax1 = min(Nk_half - iq1, Nk_half) + Nk_half + 1      (1-based).
+ * fields and out are [nfield][nomega][nk] (k fastest, the reference's
+ * dos_l(knv2, omeganum)). */
+int wto_kpath_jdos(int nk, int nomega, int nfield, const double *fields, double *out)
+{
+    if (nk < 1 || nomega < 1 || nfield < 1) return -1;
+    const int nkh = (nk - 1) / 2;
+#pragma omp parallel for schedule(static)
+    for (long fj = 0; fj < (long)nfield * nomega; ++fj) {
+        const double *a = fields + (size_t)fj * nk;
+        double *o = out + (size_t)fj * nk;
+        for (int iq = 1; iq <= nk; ++iq) {
+            const int iq1 = iq - nkh;
+            const int lo = (-nkh - iq1 > -nkh ? -nkh - iq1 : -nkh) + nkh + 1;
+            const int hi = (nkh - iq1 < nkh ? nkh - iq1 : nkh) + nkh + 1;
+            double s = 0.0;
+            for (int ik1 = lo; ik1 <= hi; ++ik1) s += a[ik1 - 1] * a[ik1 + iq1 - 1];
+            o[iq - 1] = s;
+        }
+    }
+    return 0;
+}

@@ -13,6 +13,7 @@
  *   wto_spectral_spin      <- src/surfstat.f90:102-115,163-192 /
  *                             src/fermiarc.f90:85-101,250-328
  *   wto_kplane_autocorrelation <- src/fermiarc.f90:496-560 (QPI jdos / jsdos)
+ *   wto_kpath_jdos         <- src/surfstat.f90:891-907 (surfstat_jdos)
  *
  * Conventions (all as in the reference): complex matrices are column-major,
  * interleaved (re, im) doubles, leading dimension = order; 1-based orbital
@@ -35,6 +36,8 @@
  *   wto_kplane_autocorrelation -- the reference program's arc.jdat_* /
  *                                arc.jsdat_*, and an FFT cyclic correlation
  *                                of the folded plane.
+ *   wto_kpath_jdos            -- the reference program's dos.jdat_*, and
+ *                                numpy's full linear correlation.
  */
 #ifndef WTSURF_ORACLE_H
 #define WTSURF_ORACLE_H
@@ -94,6 +97,11 @@ int wto_spectral_spin(int nw, int np, int nr, int ijmax, const int *irvec,
  * [nky][nkx], ik1 fastest. */
 int wto_kplane_autocorrelation(int nkx, int nky, int m1, int m2, int nfield,
                                const double *fields, double *out);
+
+/* k-path joint density of states of surfstat_jdos (src/surfstat.f90:891-907):
+ * out_f(iq, j) = sum_{ik1} a_f(ik1, j) a_f(ik1 + iq - (nk-1)/2, j) over the
+ * reference's index window.  fields, out [nfield][nomega][nk], k fastest. */
+int wto_kpath_jdos(int nk, int nomega, int nfield, const double *fields, double *out);
 
 #ifdef __cplusplus
 }

@@ -8,7 +8,9 @@ dos.dat_{l,r,bulk} and spindos.dat_{l,r} of surfstat (SlabSS_calc) and
 arc.dat_{l,r,bulk} of SurfaceDOSkk (SlabArc_calc) at the top; the QPI run
 (SlabQPI_kplane_calc: arc.dat_*, arc.jdat_*, with SOC arc.jsdat_* and the
 spin texture spindos.dat_*) under qpi/ and the spin texture run
-(SlabSpintexture_calc) under spintexture/, each with the wt.in it ran.
+(SlabSpintexture_calc) under spintexture/ and the k-path QPI run
+(SlabQPI_kpath_calc: surfstat_jdos's dos.dat_*, spindos.dat_*, dos.jdat_*)
+under qpi_kpath/, each with the wt.in it ran.
 
     python tests/golden/make_golden_files.py
 """
@@ -85,6 +87,9 @@ RUNS = {
                                         "spindos.dat_l", "spindos.dat_r"]),
     "spintexture": ("SlabSpintexture_calc = T", ["arc.dat_l", "arc.dat_r", "arc.dat_bulk",
                                                  "spindos.dat_l", "spindos.dat_r"]),
+    "qpi_kpath": ("SlabQPI_kpath_calc = T", ["dos.dat_l", "dos.dat_r", "dos.dat_bulk",
+                                             "spindos.dat_l", "spindos.dat_r", "dos.jdat_l",
+                                             "dos.jdat_r"]),
 }
 
 PROJ = ["s", "pz", "px", "py", "dz2", "dxz", "dyz", "dxy", "dx2-y2"]
@@ -135,7 +140,7 @@ CASES = {
     "ndim16": dict(mdl=M.model_ndim16, soc=0, nprojs=[8], np_=2, eta=0.002, earc=0.1,
                    efermi=0.05, nomega=21, nk1=11, nk2=7,
                    lattice=["1.1 0.0 0.0", "0.3 0.9 0.0", "0.0 0.1 1.3"],
-                   runs=["base", "qpi"]),
+                   runs=["base", "qpi", "qpi_kpath"]),
     # spinful: 2 atoms x (3 + 2) orbitals x spin = 10, Np = 3 (Ndim = 30),
     # SURFACE_ATOMS picks different atoms for the two surfaces
     "soc_ndim30": dict(mdl=lambda: M.synthetic_model(10, 1, 0.3, M.decay_l1, seed=11,
@@ -150,7 +155,8 @@ CASES = {
                                                          soc_sigma=0.08, name="soc10rot"),
                            soc=1, nprojs=[5], np_=2, eta=0.005, earc=0.15, efermi=0.02,
                            nomega=11, nk1=7, nk2=5,
-                           lattice=["0.9 0.3 0.1", "-0.2 1.0 0.25", "0.1 -0.1 1.2"]),
+                           lattice=["0.9 0.3 0.1", "-0.2 1.0 0.25", "0.1 -0.1 1.2"],
+                           runs=["base", "qpi_kpath"]),
     # the QPI and spin texture drivers on the model above: Nk1 even (nkx = Nk1 + 1,
     # so jdos and jsdos wrap at different moduli) and a KPLANE_SLAB whose
     # vectors QPI rounds outwards (src/fermiarc.f90:113-138)
@@ -180,7 +186,7 @@ def main() -> None:
                     run_reference(d)
                     outputs = OUTPUTS
                 else:
-                    run_wtx(d, "arc.dat_l")
+                    run_wtx(d, outputs[0])
                     outputs = [fn for fn in outputs if os.path.exists(os.path.join(d, fn))]
                 os.makedirs(sub, exist_ok=True)
                 for fn in ["wt.in", f"{name}_hr.dat"] + outputs:

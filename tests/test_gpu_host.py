@@ -26,8 +26,10 @@ WT_SURF = os.path.join(ROOT, "wannier_tools_amd", "bin", "wt_surf")
 WT_GPU = os.path.join(ROOT, "oracle", "_ref", "wt_gpu.x")
 
 
-# SurfaceDOSkk's QPI and spin texture drivers, each run in its own directory
-RUNS = [("ndim16", "qpi"), ("qpi_soc_ndim20", "qpi"), ("qpi_soc_ndim20", "spintexture")]
+# SurfaceDOSkk's QPI and spin texture drivers and surfstat_jdos, each run in its
+# own directory
+RUNS = [("ndim16", "qpi"), ("qpi_soc_ndim20", "qpi"), ("qpi_soc_ndim20", "spintexture"),
+        ("ndim16", "qpi_kpath"), ("soc_rot_ndim20", "qpi_kpath")]
 
 
 def run_case(exe, name, tmp_path, run=None):
@@ -62,6 +64,7 @@ def test_reference_program_with_gpu_drop_in(name, tmp_path):
 
 @pytest.mark.parametrize("name,run", RUNS)
 def test_wt_surf_qpi_and_spin_texture_files(name, run, tmp_path):
+    """QPI (k-plane and k-path) and spin texture runs of the standalone driver."""
     assert os.path.exists(WT_SURF), "wt_surf not built (run __graft_entry__.build())"
     run_case(WT_SURF, name, tmp_path, run)
 

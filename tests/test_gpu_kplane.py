@@ -1,6 +1,7 @@
-"""The QPI autocorrelation on the GPU (wtsurf_kplane_autocorrelation) against
-the oracle's literal restatement of the reference loops, and at a 257 x 257
-plane against the FFT cyclic correlation of the folded plane."""
+"""The QPI sums on the GPU against the oracle's literal restatements of the
+reference loops: the k-plane autocorrelation (wtsurf_kplane_autocorrelation),
+also at a 257 x 257 plane against the FFT cyclic correlation of the folded
+plane, and the k-path jdos of surfstat_jdos (wtsurf_kpath_jdos)."""
 import numpy as np
 import pytest
 
@@ -48,3 +49,16 @@ def test_autocorrelation_rejects_bad_moduli(gpu):
     for m1, m2 in ((0, 4), (6, 4), (4, 6)):
         with pytest.raises(WtsurfError):
             gpu.kplane_autocorrelation(a, m1, m2)
+
+
+@pytest.mark.parametrize("nk,nomega,nfield", [(1, 3, 1), (2, 2, 2), (22, 21, 4), (513, 7, 4),
+                                              (1024, 3, 1)])
+def test_kpath_jdos_vs_oracle(gpu, oracle_lib, nk, nomega, nfield):
+    print("kpath jdos", nk, nomega, nfield, flush=True)
+    rng = np.random.default_rng(nk + nomega)
+    a = rng.uniform(0.0, 3.0, (nfield, nk, nomega))
+    got = gpu.kpath_jdos(a)
+    want = oracle_lib.kpath_jdos(a)
+    assert got.shape == a.shape
+    # the same terms added in the same order: equal up to contraction into FMA
+    assert np.allclose(got, want, rtol=1e-13, atol=0)

@@ -1,8 +1,9 @@
 """SurfaceDOSkk's spin texture and QPI outputs (src/fermiarc.f90:113-150,
-250-562): the reference program's own files (tests/golden/files_*/qpi,
-.../spintexture, written by oracle/_ref/wt.x) against the oracle driven
-through the same conventions, and the oracle's k-plane autocorrelation
-against an FFT cyclic correlation.
+250-562) and surfstat_jdos's k-path QPI (src/surfstat.f90:606-962): the
+reference program's own files (tests/golden/files_*/qpi, .../spintexture,
+.../qpi_kpath, written by oracle/_ref/wt.x) against the oracle driven through
+the same conventions, the oracle's k-plane autocorrelation against an FFT
+cyclic correlation and its k-path jdos against numpy's linear correlation.
 
 Conventions the files pin: the outward rounding of the KPLANE_SLAB vectors
 for QPI, the max/10 threshold that also resets the spins, the eps6 row filter
@@ -146,3 +147,72 @@ def test_autocorrelation_against_fft(oracle_lib, nkx, nky, m1, m2, nfield):
     want = cyc[np.arange(nkx)[:, None] % m1, np.arange(nky)[None, :] % m2]
     got = oracle_lib.kplane_autocorrelation(a, m1, m2)
     assert np.allclose(got, want, rtol=1e-12, atol=1e-12 * np.abs(want).max())
+
+
+# ---- surfstat_jdos (SlabQPI_kpath_calc) ------------------------------------
+KPATH_CASES = {
+    "ndim16": dict(nw=8, np_=2, soc=0, nomega=21, nk1=11, efermi=0.05, top_atoms=[1],
+                   bot_atoms=[1], nprojs=[8], lattice="ndim16"),
+    "soc_rot_ndim20": dict(nw=10, np_=2, soc=1, nomega=11, nk1=7, efermi=0.02, top_atoms=[1],
+                           bot_atoms=[1], nprojs=[5], lattice="soc_rot_ndim20"),
+}
+KPATH = [((0.0, 0.0), (0.5, 0.0)), ((0.5, 0.0), (0.5, 0.5))]
+
+
+@pytest.mark.parametrize("name", list(KPATH_CASES))
+def test_oracle_reproduces_reference_surfstat_jdos_files(oracle_lib, name):
+    """surfstat_jdos (src/surfstat.f90:606-962): energies from the step
+    (OmegaMax - OmegaMin)/OmegaNum formed first, eta = 3 step, raw (unlogged)
+    traces and spin traces, and the k-path jdos of dos_l, dos_r and the
+    eps9-floored dos_*_only."""
+    c = KPATH_CASES[name]
+    d = os.path.join(GOLDEN, f"files_{name}", "qpi_kpath")
+    mdl = M.read_hr(os.path.join(d, f"{name}_hr.dat"), e_fermi=c["efermi"])
+    irvec, hr = mdl.surface_cell()
+    lo, hi = -1.0 * M.EV2HARTREE, 1.0 * M.EV2HARTREE
+    step = (hi - lo) / c["nomega"]
+    om = lo + np.arange(c["nomega"]) * step
+    kp = M.kpath_slab(KPATH, c["nk1"])
+    top = orbitals(c["top_atoms"], c["nprojs"], c["soc"])
+    bot = orbitals(c["bot_atoms"], c["nprojs"], c["soc"])
+    dl, dr, db, _, sl, sr = oracle_lib.spectral_spin(c["nw"], c["np_"], irvec, hr, kp, om,
+                                                     step * 3.0, top, bot, spin_mode=1)
+    only_l = np.where(dl - db < 0, M.EPS9, dl - db)
+    only_r = np.where(dr - db < 0, M.EPS9, dr - db)
+    jd = oracle_lib.kpath_jdos(np.stack([dl, dr, only_l, only_r]))
+
+    def check(fn, cols):
+        ref = read_table(os.path.join(d, fn))[1]
+        assert ref.shape == (kp.shape[0] * om.size, 2 + len(cols)), fn
+        assert np.allclose(ref[:, 1], np.tile(om / M.EV2HARTREE, kp.shape[0]), rtol=1e-7, atol=1e-8)
+        for i, col in enumerate(cols):
+            got = col.ravel()  # (nk, nomega): k outer, omega inner, as the rows
+            assert np.allclose(got, ref[:, 2 + i], rtol=2e-7, atol=1e-300), (fn, i)
+
+    check("dos.dat_l", [dl, only_l])
+    check("dos.dat_r", [dr, only_r])
+    check("dos.dat_bulk", [db])
+    check("dos.jdat_l", [jd[0], jd[2]])
+    check("dos.jdat_r", [jd[1], jd[3]])
+    urot = surface_rotation(*LATTICE[c["lattice"]][:2])
+    for side, s in (("l", sl), ("r", sr)):
+        ref = read_table(os.path.join(d, f"spindos.dat_{side}"))[1][:, 2:]
+        got = np.einsum("ij,jkw->kwi", urot, s).reshape(-1, 3)
+        assert np.all(np.abs(got - ref) <= 2e-7 * np.abs(ref) + 2e-9 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("nk", [1, 2, 3, 8, 11])
+def test_kpath_jdos_against_numpy_correlation(oracle_lib, nk):
+    """out(iq) is the linear autocorrelation of the first 2h + 1 points
+    (h = (nk - 1)/2) at lag iq - h (1-based iq), zero beyond the window."""
+    rng = np.random.default_rng(nk)
+    a = rng.uniform(0.1, 1.0, (nk, 3))
+    got = oracle_lib.kpath_jdos(a)
+    h = (nk - 1) // 2
+    for j in range(3):
+        b = a[: 2 * h + 1, j]
+        full = np.correlate(b, b, mode="full")  # full[lag + 2h] = sum_n b[n + lag] b[n]
+        for iq0 in range(nk):
+            lag = iq0 + 1 - h
+            want = full[lag + 2 * h] if abs(lag) <= 2 * h else 0.0
+            assert abs(got[iq0, j] - want) <= 1e-13 * abs(full).max()

@@ -1,4 +1,6 @@
-// QPI autocorrelation over a k-plane on gfx950 (SurfaceDOSkk's jdos / jsdos,
+// The two QPI sums on gfx950.
+//
+// 1. Autocorrelation over a k-plane (SurfaceDOSkk's jdos / jsdos,
 // src/fermiarc.f90:496-560):
 //
 //   out(iq1, iq2) = sum_{ik1, ik2} sum_f a_f(ik1, ik2) a_f(ik1q, ik2q)
@@ -17,6 +19,13 @@
 // c1 + p1 needs no wrap) in LDS; every thread then reads the E row at one
 // address (broadcast) and the a row at consecutive addresses.  The sum is
 // O((m1 m2)^2) FP64 multiply-adds, milliseconds at a 257 x 257 plane.
+//
+// 2. Linear autocorrelation along a k path, per energy (surfstat_jdos,
+// src/surfstat.f90:891-907): with h = (nk - 1)/2 and s = iq - h,
+//
+//   out(iq, j) = sum_{ik = max(1, 1 - s)}^{min(2h + 1, 2h + 1 - s)} a(ik, j) a(ik + s, j)
+//
+// one thread per (iq, j, field), the terms added in the reference's order.
 #include <algorithm>
 
 #include "wtsurf_device.h"
@@ -99,9 +108,38 @@ __global__ void __launch_bounds__(kCorrBlock) expand_kernel(const double *__rest
     }
 }
 
+// out_f(iq, j), fields and out [nfield][nomega][nk]; thread iq of row (j, f).
+__global__ void __launch_bounds__(kCorrBlock) kpath_jdos_kernel(const double *__restrict__ a,
+                                                                int nk, double *__restrict__ out) {
+    const int iq = blockIdx.x * kCorrBlock + threadIdx.x;  // 0-based
+    if (iq >= nk) return;
+    const size_t row = (size_t)blockIdx.y * nk;
+    const double *r = a + row;
+    const int h = (nk - 1) / 2;
+    const int sft = iq + 1 - h;
+    const int lo = sft < 0 ? -sft : 0;                       // 0-based first ik
+    const int hi = sft > 0 ? 2 * h - sft : 2 * h;            // 0-based last ik
+    double s = 0.0;
+    for (int ik = lo; ik <= hi; ++ik) s += r[ik] * r[ik + sft];
+    out[row + iq] = s;
+}
+
 }  // namespace
 
 int kplane_corr_max_m() { return kKplaneMaxM; }
+
+int kpath_jdos_launch(const double *fields, int nk, int nomega, int nfield, double *out,
+                      hipStream_t s) {
+    if (nk < 1 || nomega < 1 || nfield < 1 || (long)nomega * nfield > 65535L * 1024) return -1;
+    const long rows = (long)nomega * nfield;
+    for (long r0 = 0; r0 < rows; r0 += 65535) {
+        const int nr = (int)std::min<long>(rows - r0, 65535);
+        hipLaunchKernelGGL(kpath_jdos_kernel, dim3((nk + kCorrBlock - 1) / kCorrBlock, nr),
+                           dim3(kCorrBlock), 0, s, fields + (size_t)r0 * nk, nk,
+                           out + (size_t)r0 * nk);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -2;
+}
 
 int kplane_corr_launch(const KplaneCorrArgs &k, hipStream_t s) {
     if (k.nkx < 1 || k.nky < 1 || k.m1 < 1 || k.m2 < 1 || k.m1 > k.nkx || k.m2 > k.nky ||

@@ -588,6 +588,30 @@ int wtsurf_kplane_autocorrelation(wtsurf_ctx *c, int nkx, int nky, int m1, int m
     return rc;
 }
 
+int wtsurf_kpath_jdos(wtsurf_ctx *c, int nk, int nomega, int nfield, const double *fields,
+                      double *out) {
+    if (!c) return -1;
+    if (nk < 1 || nomega < 1 || nfield < 1 || !fields || !out) return c->fail("invalid arguments");
+    if (int r = use_device(c)) return r;
+    const size_t n = (size_t)nfield * nomega * nk;
+    DevBuf<double> buf;
+    if (buf.ensure(2 * n)) return c->fail("out of device memory (k-path jdos)");
+    int rc = 0;
+    hipError_t e;
+    if ((e = hipMemcpyAsync(buf.p, fields, n * sizeof(double), hipMemcpyHostToDevice, c->stream)) !=
+        hipSuccess)
+        rc = c->hipfail(e, "upload");
+    else if (kpath_jdos_launch(buf.p, nk, nomega, nfield, buf.p + n, c->stream))
+        rc = c->fail("k-path jdos launch failed");
+    else if ((e = hipMemcpyAsync(out, buf.p + n, n * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream)) != hipSuccess ||
+             (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        rc = c->hipfail(e, "download");
+    if (rc) (void)hipStreamSynchronize(c->stream);
+    buf.release();
+    return rc;
+}
+
 int wtsurf_spectral_grid(wtsurf_ctx *c, int nk, const double *kpts, int nomega, const double *omega,
                          double eta, double divisor, int precision, double *dos_l, double *dos_r,
                          double *dos_bulk, int *iters, int *status) {

@@ -100,5 +100,9 @@ struct KplaneCorrArgs {
 };
 int kplane_corr_max_m();
 int kplane_corr_launch(const KplaneCorrArgs &k, hipStream_t s);
+// k-path joint density of states (src/surfstat.f90:891-907); device buffers
+// [nfield][nomega][nk].
+int kpath_jdos_launch(const double *fields, int nk, int nomega, int nfield, double *out,
+                      hipStream_t s);
 
 }  // namespace wtsurf

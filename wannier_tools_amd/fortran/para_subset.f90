@@ -38,7 +38,7 @@ module para
    character(80) :: Particle = 'electron'
    logical :: SlabSS_calc = .false., SlabArc_calc = .false.
    logical :: SlabSpintexture_calc = .false., SlabQPI_kplane_calc = .false.
-   logical :: SlabQPI_calc = .false.
+   logical :: SlabQPI_calc = .false., SlabQPI_kpath_calc = .false.
    logical :: LOTO_correction = .false.
    integer :: Soc = 0
    real(Dp) :: E_fermi = 0d0

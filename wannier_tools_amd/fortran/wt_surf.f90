@@ -2,7 +2,8 @@
 !
 ! Run in a WannierTools input directory: reads wt.in (the part the path
 ! uses) and the Wannier90 *_hr.dat it names, then runs SlabSS_calc (surfstat,
-! dos.dat_*, spindos.dat_*), SlabArc_calc, SlabQPI_calc / SlabQPI_kplane_calc
+! dos.dat_*, spindos.dat_*), SlabQPI_kpath_calc (surfstat_jdos, dos.jdat_*),
+! SlabArc_calc, SlabQPI_calc / SlabQPI_kplane_calc
 ! and SlabSpintexture_calc (SurfaceDOSkk: arc.dat_*, arc.jdat_*, arc.jsdat_*,
 ! spindos.dat_*) exactly as the reference's main program dispatches them
 ! (src/main.f90:714-765), and reports the time of each in WT.out.
@@ -10,7 +11,8 @@
 ! Input subset (names, defaults and units as src/readinput.f90):
 !   &TB_FILE    Hrfile, Particle
 !   &CONTROL    SlabSS_calc, SlabArc_calc, SlabQPI_calc, SlabQPI_kplane_calc,
-!               SlabSpintexture_calc (other flags are accepted and ignored)
+!               SlabQPI_kpath_calc, SlabSpintexture_calc (other flags are accepted
+!               and ignored)
 !   &SYSTEM     SOC, E_FERMI
 !   &PARAMETERS Eta_Arc, E_arc, OmegaNum, OmegaMin, OmegaMax, Nk1, Nk2, NP
 !   cards       LATTICE, ATOM_POSITIONS, PROJECTORS, SURFACE, KPATH_SLAB,
@@ -260,6 +262,7 @@ contains
       call get_logical('SlabSpintexture_calc', SlabSpintexture_calc)
       call get_logical('SlabQPI_kplane_calc', SlabQPI_kplane_calc)
       call get_logical('SlabQPI_calc', SlabQPI_calc)
+      call get_logical('SlabQPI_kpath_calc', SlabQPI_kpath_calc)
       SlabQPI_kplane_calc = SlabQPI_kplane_calc .or. SlabQPI_calc  ! src/readinput.f90:171
       call get_logical('LOTO_correction', LOTO_correction)
       call get_int('SOC', Soc)
@@ -367,7 +370,7 @@ contains
                k2len(i + (j - 1)*Nk1) = t1
             end do
          end do
-      else if (SlabSS_calc) then
+      else if (SlabSS_calc .or. SlabQPI_kpath_calc) then
          stop 'ERROR: please set KPATH_SLAB for slab band structure calculation'
       end if
 
@@ -519,6 +522,14 @@ program wt_surf
       call SurfaceDOSkk
       call system_clock(c1)
       write(stdout, '(a, f16.3, a)') ' Time cost for SlabArc is about ', dble(c1 - c0)/rate, ' s'
+   end if
+   if (SlabQPI_kpath_calc) then
+      write(stdout, *) ' '
+      write(stdout, *) 'Start to calculate the surface QPI in kpath mode'
+      call system_clock(c0, rate)
+      call surfstat_jdos
+      call system_clock(c1)
+      write(stdout, '(a, f16.3, a)') ' Time cost for SlabQPI is about ', dble(c1 - c0)/rate, ' s'
    end if
    if (SlabQPI_calc .or. SlabQPI_kplane_calc) then
       write(stdout, *) ' '

@@ -110,6 +110,15 @@ module wtsurf_cabi
          real(c_double), intent(out) :: out(*)
       end function
 
+      integer(c_int) function wtsurf_kpath_jdos(ctx, nk, nomega, nfield, fields, out) &
+            bind(C, name='wtsurf_kpath_jdos')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: nk, nomega, nfield
+         real(c_double), intent(in) :: fields(*)
+         real(c_double), intent(out) :: out(*)
+      end function
+
       integer(c_int) function wtsurf_max_ndim() bind(C, name='wtsurf_max_ndim')
          import :: c_int
       end function

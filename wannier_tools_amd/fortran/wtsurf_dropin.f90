@@ -8,7 +8,8 @@
 !   surfgreen_1985(omega, GLL, GRR, GB, H00, H01, ones, eta_broadening)
 !                                             src/surfgreen.f90:7-137
 !   surfstat                                  src/surfstat.f90:1-605
-!   SurfaceDOSkk                              src/fermiarc.f90:1-494
+!   surfstat_jdos                             src/surfstat.f90:606-962
+!   SurfaceDOSkk                              src/fermiarc.f90:1-994
 !
 ! ham_qlayer2qlayer and surfgreen_1985 are per-call, for callers that loop
 ! themselves.  surfstat and SurfaceDOSkk submit their whole (k, omega) grid to
@@ -259,6 +260,140 @@ subroutine surfstat
 2002 format(4(1X, E16.8))
 2003 format(3(1X, E16.8))
 end subroutine surfstat
+
+
+!> Surface spectral function and its k-path joint density of states
+!> (src/surfstat.f90:606-962): omega_j = OmegaMin + (j-1) dOmega with
+!> dOmega = (OmegaMax-OmegaMin)/OmegaNum, eta = 3 dOmega, the surfstat traces
+!> and spin traces written raw (no log, no header) to dos.dat_* and
+!> spindos.dat_*, and the jdos of dos_l, dos_r, dos_l_only, dos_r_only along the
+!> path, computed on the GPU, to dos.jdat_l, dos.jdat_r.
+subroutine surfstat_jdos
+   use wmpi
+   use para
+   use wtsurf_gpu
+   use wtsurf_cabi
+   implicit none
+   integer :: i, j, ikp, doslfile, dosrfile, dosbulkfile, spindoslfile, spindosrfile
+   integer :: jdoslfile, jdosrfile
+   real(dp) :: eta_broadening, s0(3), s1(3)
+   real(dp), allocatable :: omega(:), kpts(:, :)
+   real(dp), allocatable :: dos_l(:, :), dos_r(:, :), dos_bulk(:, :)
+   real(dp), allocatable :: spin_l(:, :, :), spin_r(:, :, :)
+   real(dp), allocatable :: fields(:, :, :), jdos(:, :, :)
+   integer(c_int), allocatable :: iters(:, :), status(:, :)
+
+   if (cpuid /= 0) return
+
+   allocate(omega(omeganum), kpts(2, knv2))
+   allocate(dos_l(knv2, omeganum), dos_r(knv2, omeganum), dos_bulk(knv2, omeganum))
+   allocate(spin_l(knv2, omeganum, 3), spin_r(knv2, omeganum, 3))
+   allocate(fields(knv2, omeganum, 4), jdos(knv2, omeganum, 4))
+   allocate(iters(knv2, omeganum), status(knv2, omeganum))
+
+   ! src/surfstat.f90:691-696 (the step is formed first, unlike surfstat)
+   eta_broadening = (omegamax - omegamin)/dble(omeganum)
+   do i = 1, omeganum
+      omega(i) = omegamin + (i - 1)*eta_broadening
+   end do
+   eta_broadening = eta_broadening*3.0d0
+   do ikp = 1, knv2
+      kpts(:, ikp) = k2_path(ikp, :)
+   end do
+
+   call wtsurf_gpu_upload_model(.true.)
+   call wtsurf_gpu_check(wtsurf_set_spin_traces(wtsurf_gpu_ctx(), WTSURF_SPIN_SURFSTAT), &
+      'wtsurf_set_spin_traces')
+   call wtsurf_gpu_check(wtsurf_spectral_grid(wtsurf_gpu_ctx(), int(knv2, c_int), kpts, &
+      int(omeganum, c_int), omega, eta_broadening, 1.0d0, wtsurf_gpu_precision, &
+      dos_l, dos_r, dos_bulk, iters, status), 'wtsurf_spectral_grid')
+   call wtsurf_gpu_check(wtsurf_spin_download(wtsurf_gpu_ctx(), spin_l, spin_r), &
+      'wtsurf_spin_download')
+   call wtsurf_gpu_check(wtsurf_set_spin_traces(wtsurf_gpu_ctx(), WTSURF_SPIN_OFF), &
+      'wtsurf_set_spin_traces')
+   if (any(iand(status, WTSURF_STATUS_SINGULAR) /= 0)) print *, 'something wrong with zgesv'
+
+   ! src/surfstat.f90:838-845
+   fields(:, :, 1) = dos_l
+   fields(:, :, 2) = dos_r
+   do ikp = 1, knv2
+      do j = 1, omeganum
+         fields(ikp, j, 3) = dos_l(ikp, j) - dos_bulk(ikp, j)
+         if (fields(ikp, j, 3) < 0) fields(ikp, j, 3) = eps9
+         fields(ikp, j, 4) = dos_r(ikp, j) - dos_bulk(ikp, j)
+         if (fields(ikp, j, 4) < 0) fields(ikp, j, 4) = eps9
+      end do
+   end do
+
+   outfileindex = outfileindex + 1
+   doslfile = outfileindex
+   outfileindex = outfileindex + 1
+   dosrfile = outfileindex
+   outfileindex = outfileindex + 1
+   dosbulkfile = outfileindex
+   outfileindex = outfileindex + 1
+   spindoslfile = outfileindex
+   outfileindex = outfileindex + 1
+   spindosrfile = outfileindex
+   open(unit=doslfile, file='dos.dat_l')
+   open(unit=dosrfile, file='dos.dat_r')
+   open(unit=dosbulkfile, file='dos.dat_bulk')
+   open(unit=spindoslfile, file='spindos.dat_l')
+   open(unit=spindosrfile, file='spindos.dat_r')
+   do ikp = 1, knv2
+      do j = 1, omeganum
+         write(doslfile, 2002) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_l(ikp, j), fields(ikp, j, 3)
+         write(dosrfile, 2002) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_r(ikp, j), fields(ikp, j, 4)
+         write(dosbulkfile, 2003) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, dos_bulk(ikp, j)
+         s0 = spin_l(ikp, j, :)
+         call rotate(s0, s1)
+         write(spindoslfile, 2001) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, s1
+         s0 = spin_r(ikp, j, :)
+         call rotate(s0, s1)
+         write(spindosrfile, 2001) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, s1
+      end do
+      write(doslfile, *)
+      write(dosrfile, *)
+      write(dosbulkfile, *)
+      write(spindoslfile, *)
+      write(spindosrfile, *)
+   end do
+   close(doslfile)
+   close(dosrfile)
+   close(dosbulkfile)
+   close(spindoslfile)
+   close(spindosrfile)
+
+   ! jdos of dos_l, dos_r, dos_l_only, dos_r_only along the path (src/surfstat.f90:891-907)
+   call wtsurf_gpu_check(wtsurf_kpath_jdos(wtsurf_gpu_ctx(), int(knv2, c_int), &
+      int(omeganum, c_int), 4_c_int, fields, jdos), 'wtsurf_kpath_jdos')
+
+   outfileindex = outfileindex + 1
+   jdoslfile = outfileindex
+   outfileindex = outfileindex + 1
+   jdosrfile = outfileindex
+   open(unit=jdoslfile, file='dos.jdat_l')
+   open(unit=jdosrfile, file='dos.jdat_r')
+   ! both headers name jdos_l, as the reference writes them (src/surfstat.f90:935-936)
+   write(jdoslfile, '("#", a12, 3a17)') ' k(1/Ang)', ' E(eV)', 'jdos_l', 'jdos_l_only'
+   write(jdosrfile, '("#", a12, 3a17)') ' k(1/Ang)', ' E(eV)', 'jdos_l', 'jdos_l_only'
+   do ikp = 1, knv2
+      do j = 1, omeganum
+         write(jdoslfile, 2002) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, jdos(ikp, j, 1), jdos(ikp, j, 3)
+         write(jdosrfile, 2002) k2len(ikp)*Angstrom2atomic, omega(j)/eV2Hartree, jdos(ikp, j, 2), jdos(ikp, j, 4)
+      end do
+      write(jdoslfile, *)
+      write(jdosrfile, *)
+   end do
+   close(jdoslfile)
+   close(jdosrfile)
+   write(stdout, *)
+   write(stdout, *) 'calculate joint density of state successfully'
+
+2001 format(5(1X, E16.8))
+2002 format(4(1X, E16.8))
+2003 format(3(1X, E16.8))
+end subroutine surfstat_jdos
 
 
 !> Surface spectral function on KPLANE_SLAB at iso_energy (src/fermiarc.f90:1-494)

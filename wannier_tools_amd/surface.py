@@ -14,6 +14,8 @@ reference's Fortran routines:
   (src/fermiarc.f90:250-328)
 * ``SurfaceGF.kplane_autocorrelation`` -- the QPI jdos / jsdos sums of
   SurfaceDOSkk (src/fermiarc.f90:496-560)
+* ``SurfaceGF.kpath_jdos`` -- the k-path jdos of surfstat_jdos
+  (src/surfstat.f90:891-907)
 
 There is no CPU fallback: constructing a ``SurfaceGF`` without the built
 library, or without a HIP device, raises.
@@ -68,6 +70,7 @@ SIGNATURES = {
     "wtsurf_spin_download": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "wtsurf_kplane_autocorrelation": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 5
                                       + [_dp, _dp]),
+    "wtsurf_kpath_jdos": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 3 + [_dp, _dp]),
     "wtsurf_max_ndim": (ctypes.c_int, []),
 }
 
@@ -279,6 +282,22 @@ class SurfaceGF:
                                                            nfield, _dptr(buf), _dptr(out)),
                     "wtsurf_kplane_autocorrelation")
         return out.T.copy()
+
+    def kpath_jdos(self, fields) -> np.ndarray:
+        """k-path joint density of states (see include/wtsurf.h): fields
+        (nfield, nk, nomega) or (nk, nomega) indexed [f, ik-1, j-1]; returns the
+        same shape."""
+        f = np.asarray(fields, dtype=np.float64)
+        single = f.ndim == 2
+        if single:
+            f = f[None]
+        nfield, nk, nom = f.shape
+        buf = np.ascontiguousarray(f.transpose(0, 2, 1))
+        out = np.zeros_like(buf)
+        self._check(self.lib.wtsurf_kpath_jdos(self.ctx, nk, nom, nfield, _dptr(buf), _dptr(out)),
+                    "wtsurf_kpath_jdos")
+        out = out.transpose(0, 2, 1).copy()
+        return out[0] if single else out
 
     # ---- resident split (inputs stay in device memory) -------------------
     def upload_grid(self, kpts, omegas) -> None:
