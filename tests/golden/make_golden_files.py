@@ -68,8 +68,7 @@ PROJECTORS
 {nprojs}
 {projlines}
 SURFACE
-1 0 0
-0 1 0
+{surface}
 KPATH_SLAB
 2
 G 0.0 0.0 X 0.5 0.0
@@ -98,12 +97,15 @@ PROJ = ["s", "pz", "px", "py", "dz2", "dxz", "dyz", "dxy", "dx2-y2"]
 def write_case(d: str, name: str, mdl: M.TBModel, soc: int, nprojs: list[int], np_: int,
                eta: float, earc: float, efermi: float, nomega: int, nk1: int, nk2: int,
                lattice, surface_atoms: str = "", control: str = RUNS["base"][0],
-               kplane: str = KPLANE_DEFAULT) -> None:
+               kplane: str = KPLANE_DEFAULT, surface: str = "1 0 0\n0 1 0",
+               positions=None) -> None:
     hrname = f"{name}_hr.dat"
     M.write_hr(os.path.join(d, hrname), mdl)
-    atoms = "\n".join(f"C {0.1 * i:.3f} {0.05 * i:.3f} {0.2 * i:.3f}" for i in range(len(nprojs)))
+    if positions is None:
+        positions = [(0.1 * i, 0.05 * i, 0.2 * i) for i in range(len(nprojs))]
+    atoms = "\n".join(f"C {x:.3f} {y:.3f} {z:.3f}" for x, y, z in positions)
     projlines = "\n".join("C " + " ".join(PROJ[:p]) for p in nprojs)
-    text = WT_IN.format(control=control, kplane=kplane,
+    text = WT_IN.format(control=control, kplane=kplane, surface=surface,
                         hr=hrname, soc=soc, efermi=efermi, eta=eta, earc=earc, nomega=nomega,
                         nk1=nk1, nk2=nk2, np=np_, a1=lattice[0], a2=lattice[1], a3=lattice[2],
                         natoms=len(nprojs), atoms=atoms, nprojs=" ".join(map(str, nprojs)),
@@ -167,6 +169,17 @@ CASES = {
                            lattice=["0.9 0.3 0.1", "-0.2 1.0 0.25", "0.1 -0.1 1.2"],
                            kplane="-0.4 -0.3\n0.8 0.1\n-0.2 0.7",
                            runs=["qpi", "spintexture"]),
+    # the surface-cell remap (get_hmnr_cell): three atoms in a skewed cell, SURFACE
+    # rows (1 1 0) and (0 0 1) with no third row, so the reference searches for
+    # R3' (FindTheThirdLatticeVector) and hoppings move to new lattice vectors
+    "remap_soc_ndim20": dict(mdl=lambda: M.synthetic_model(10, 1, 0.35, M.decay_l1, seed=14,
+                                                           soc_sigma=0.06, name="remap10"),
+                             soc=1, nprojs=[2, 1, 2], np_=2, eta=0.005, earc=0.1, efermi=0.0,
+                             nomega=11, nk1=7, nk2=5,
+                             lattice=["1.0 0.1 0.0", "0.2 1.1 0.0", "0.1 0.2 1.3"],
+                             positions=[(0.1, 0.2, 0.3), (0.6, 0.55, 0.75), (0.35, 0.8, 0.1)],
+                             surface="1 1 0\n0 0 1",
+                             runs=["base"]),
 }
 
 
@@ -181,7 +194,8 @@ def main() -> None:
             with tempfile.TemporaryDirectory() as d:
                 write_case(d, name, c["mdl"](), c["soc"], c["nprojs"], c["np_"], c["eta"],
                            c["earc"], c["efermi"], c["nomega"], c["nk1"], c["nk2"], c["lattice"],
-                           c.get("surface_atoms", ""), control, c.get("kplane", KPLANE_DEFAULT))
+                           c.get("surface_atoms", ""), control, c.get("kplane", KPLANE_DEFAULT),
+                           c.get("surface", "1 0 0\n0 1 0"), c.get("positions"))
                 if run == "base":
                     run_reference(d)
                     outputs = OUTPUTS

@@ -19,7 +19,7 @@ from helpers import compare_tables
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["ndim16", "soc_ndim30", "soc_rot_ndim20"]
+CASES = ["ndim16", "soc_ndim30", "soc_rot_ndim20", "remap_soc_ndim20"]
 FILES = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk",
          "spindos.dat_l", "spindos.dat_r"]
 WT_SURF = os.path.join(ROOT, "wannier_tools_amd", "bin", "wt_surf")

@@ -1,0 +1,156 @@
+"""The surface-cell remap (wannier_tools_amd.cell: the SURFACE card's U, the
+new cell's atom positions, get_hmnr_cell, the surface frame) against the
+reference program's files for a three-atom spinful model whose SURFACE rows
+(1 1 0) and (0 0 1) leave the reference to search for R3'
+(tests/golden/files_remap_soc_ndim20, written by oracle/_ref/wt.x).  The
+remapped H(R) drives the oracle; every printed trace must match."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from helpers import read_table
+from test_host_files import KPATH, orbitals
+from wannier_tools_amd import cell as C
+from wannier_tools_amd import model as M
+
+NAME = "remap_soc_ndim20"
+CASE = dict(nw=10, np_=2, nprojs=[2, 1, 2], soc=True, nomega=11, nk1=7, nk2=5, eta_arc=0.005,
+            e_arc=0.1, efermi=0.0,
+            lattice=[[1.0, 0.1, 0.0], [0.2, 1.1, 0.0], [0.1, 0.2, 1.3]],
+            positions=[(0.1, 0.2, 0.3), (0.6, 0.55, 0.75), (0.35, 0.8, 0.1)],
+            surface=[[1, 1, 0], [0, 0, 1]])
+ANGSTROM2ATOMIC = 1.0 / 0.529177210903
+
+
+def remapped():
+    d = os.path.join(GOLDEN, f"files_{NAME}")
+    mdl = M.read_hr(os.path.join(d, f"{NAME}_hr.dat"), e_fermi=CASE["efermi"])
+    hr = mdl.hr.astype(np.complex128, copy=True)
+    zero = np.all(mdl.irvec == 0, axis=1)
+    for ir in np.nonzero(zero)[0]:
+        hr[ir][np.diag_indices(mdl.num_wann)] -= mdl.e_fermi
+    hr *= M.EV2HARTREE
+    lat = np.array(CASE["lattice"]) * ANGSTROM2ATOMIC
+    return d, C.build_surface_cell(lat, np.array(CASE["positions"]), CASE["nprojs"], CASE["soc"],
+                                   mdl.irvec, hr, mdl.ndegen, CASE["surface"])
+
+
+def test_surface_matrix_search_and_positions():
+    """U from the search, the shifted positions and the frame, as the
+    reference printed them for this case (POSCAR-SURFACE, WT.out)."""
+    _, sc = remapped()
+    assert np.array_equal(sc.umatrix, [[1, 1, 0], [0, 0, 1], [1, 0, 0]])
+    lat = np.array(CASE["lattice"]) * ANGSTROM2ATOMIC
+    pos = C.surface_cell_positions(sc.umatrix, np.array(CASE["positions"]))
+    assert np.allclose(pos, [[0.2, 0.3, 0.95], [0.55, 0.75, 0.1], [0.8, 0.1, 0.6]], atol=1e-12)
+    assert np.allclose(sc.ka2 / ANGSTROM2ATOMIC, [1.0368, -0.1689], atol=1e-4)
+    assert np.allclose(sc.kb2 / ANGSTROM2ATOMIC, [0.0, 1.3514], atol=1e-4)
+    assert abs(np.linalg.det(sc.umatrix @ lat) - np.linalg.det(lat)) < 1e-9
+
+
+def test_remap_preserves_every_hopping_once_per_target():
+    """Each (R, i, j) lands on exactly one (R', i, j); with no collisions the
+    remapped blocks hold the same multiset of matrix elements."""
+    d, sc = remapped()
+    mdl = M.read_hr(os.path.join(d, f"{NAME}_hr.dat"))
+    vals = np.sort(np.abs(mdl.hr[np.abs(mdl.hr) > 0]) * M.EV2HARTREE)
+    got = np.sort(np.abs(sc.hr[np.abs(sc.hr) > 0]))
+    assert got.size == vals.size
+    assert np.allclose(got, vals, rtol=1e-14)
+
+
+def test_oracle_on_remapped_model_reproduces_reference_files(oracle_lib):
+    d, sc = remapped()
+    nw, np_ = CASE["nw"], CASE["np_"]
+    top = orbitals([1, 2, 3], CASE["nprojs"], 1)
+    lo, hi = -1.0 * M.EV2HARTREE, 1.0 * M.EV2HARTREE
+    om = M.omega_grid(lo, hi, CASE["nomega"])
+    eta = (hi - lo) / CASE["nomega"] * 3.0
+    kp = M.kpath_slab(KPATH, CASE["nk1"])
+    dl, dr, db, _, sl, sr = oracle_lib.spectral_spin(nw, np_, sc.irvec, sc.hr, kp, om, eta, top, top,
+                                                     spin_mode=1)
+    for side, v in (("l", np.log(np.abs(dl))), ("r", np.log(np.abs(dr))),
+                    ("bulk", np.log(np.abs(db) + M.EPS9))):
+        ref = read_table(os.path.join(d, f"dos.dat_{side}"))[1]
+        assert np.allclose(v.ravel(), ref[:, 2], rtol=2e-7, atol=2e-7), side
+    for side, s in (("l", sl), ("r", sr)):
+        ref = read_table(os.path.join(d, f"spindos.dat_{side}"))[1][:, 2:]
+        got = np.einsum("ij,jkw->kwi", sc.urot, s).reshape(-1, 3)
+        assert np.all(np.abs(got - ref) <= 2e-7 * np.abs(ref) + 2e-9 * np.abs(ref).max()), side
+    k12, nkx, nky = M.kplane_slab((-0.5, -0.5), (1.0, 0.0), (0.0, 1.0), CASE["nk1"], CASE["nk2"])
+    dl, dr, db, _ = oracle_lib.spectral(nw, np_, sc.irvec, sc.hr, k12, [CASE["e_arc"] * M.EV2HARTREE],
+                                        CASE["eta_arc"] * M.EV2HARTREE, top, top, divisor=M.REF_PI)
+    for side, v in (("l", np.log(dl[:, 0])), ("r", np.log(dr[:, 0])), ("bulk", np.log(np.abs(db[:, 0])))):
+        ref = read_table(os.path.join(d, f"arc.dat_{side}"))[1]
+        assert np.allclose(v, ref[:, 2], rtol=0, atol=2e-8 + 2e-8 * np.abs(ref[:, 2]).max()), side
+        # the k coordinates follow the new cell's surface reciprocal vectors
+        shape = (k12[:, :1] * sc.ka2 + k12[:, 1:] * sc.kb2) * ANGSTROM2ATOMIC
+        assert np.allclose(shape, ref[:, :2], atol=2e-8)
+
+
+def test_identity_surface_leaves_the_model_alone():
+    d, _ = remapped()
+    mdl = M.read_hr(os.path.join(d, f"{NAME}_hr.dat"))
+    lat = np.array(CASE["lattice"]) * ANGSTROM2ATOMIC
+    sc = C.build_surface_cell(lat, np.array(CASE["positions"]), CASE["nprojs"], True, mdl.irvec,
+                              mdl.hr, mdl.ndegen)
+    irvec, hr = M.TBModel(mdl.num_wann, mdl.irvec, mdl.hr, mdl.ndegen).surface_cell()
+    lookup = {tuple(r): i for i, r in enumerate(sc.irvec)}
+    for ir, r in enumerate(irvec):
+        assert np.allclose(sc.hr[lookup[tuple(r)]], hr[ir] / M.EV2HARTREE)
+
+
+WT_SURF = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "wannier_tools_amd", "bin", "wt_surf")
+
+
+def read_dump(path):
+    with open(path) as f:
+        lines = [ln for ln in f if not ln.startswith("#")]
+    urot = np.array([[float(x) for x in lines[i].split()] for i in range(3)])
+    ka2 = np.array([float(x) for x in lines[3].split()])
+    kb2 = np.array([float(x) for x in lines[4].split()])
+    nw, np_, nr = (int(x) for x in lines[5].split())
+    top = np.array([int(x) for x in lines[6].split()])
+    bot = np.array([int(x) for x in lines[7].split()])
+    rows = np.array([[float(x) for x in ln.split()] for ln in lines[8:]])
+    irvec = rows[:: nw * nw, :3].astype(np.int32)
+    hr = (rows[:, 5] + 1j * rows[:, 6]).reshape(nr, nw, nw).transpose(0, 2, 1)  # i fastest
+    return dict(urot=urot, ka2=ka2, kb2=kb2, nw=nw, np=np_, irvec=irvec, hr=hr, top=top, bot=bot)
+
+
+@pytest.mark.skipif(not os.path.exists(WT_SURF), reason="wt_surf not built")
+@pytest.mark.parametrize("name,lat,pos,nprojs,soc,surface,efermi", [
+    (NAME, CASE["lattice"], CASE["positions"], CASE["nprojs"], True, CASE["surface"], 0.0),
+    ("soc_ndim30", [[1, 0, 0], [0, 1, 0], [0, 0, 1]], [(0, 0, 0), (0.1, 0.05, 0.2)], [3, 2], True,
+     [[1, 0, 0], [0, 1, 0]], 0.0),
+    ("ndim16", [[1.1, 0, 0], [0.3, 0.9, 0], [0, 0.1, 1.3]], [(0, 0, 0)], [8], False,
+     [[1, 0, 0], [0, 1, 0]], 0.05),
+])
+def test_fortran_host_remap_matches(tmp_path, name, lat, pos, nprojs, soc, surface, efermi):
+    """wt_surf --dump-cell (the Fortran host's U search, positions, remap and
+    frame; no device work) against wannier_tools_amd.cell on the same inputs."""
+    import shutil
+    import subprocess
+    d = os.path.join(GOLDEN, f"files_{name}")
+    for fn in ("wt.in", f"{name}_hr.dat"):
+        shutil.copy(os.path.join(d, fn), tmp_path / fn)
+    r = subprocess.run([WT_SURF, "--dump-cell"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_dump(tmp_path / "surface_cell.dat")
+    mdl = M.read_hr(os.path.join(d, f"{name}_hr.dat"))
+    hr = mdl.hr.astype(np.complex128, copy=True)
+    zero = np.all(mdl.irvec == 0, axis=1)
+    for ir in np.nonzero(zero)[0]:
+        hr[ir][np.diag_indices(mdl.num_wann)] -= efermi
+    hr *= M.EV2HARTREE
+    want = C.build_surface_cell(np.array(lat, float) * ANGSTROM2ATOMIC, np.array(pos, float), nprojs,
+                                soc, mdl.irvec, hr, mdl.ndegen, surface)
+    assert np.array_equal(got["irvec"], want.irvec)
+    assert np.allclose(got["hr"], want.hr, rtol=1e-15, atol=1e-18)
+    assert np.allclose(got["urot"], want.urot, atol=1e-15)
+    assert np.allclose(got["ka2"], want.ka2, rtol=1e-14, atol=1e-15)
+    assert np.allclose(got["kb2"], want.kb2, rtol=1e-14, atol=1e-15)

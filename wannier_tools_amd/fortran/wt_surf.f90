@@ -14,17 +14,270 @@
 !               SlabQPI_kpath_calc, SlabSpintexture_calc (other flags are accepted
 !               and ignored)
 !   &SYSTEM     SOC, E_FERMI
-!   &PARAMETERS Eta_Arc, E_arc, OmegaNum, OmegaMin, OmegaMax, Nk1, Nk2, NP
+!   &PARAMETERS Eta_Arc, E_arc, OmegaNum, OmegaMin, OmegaMax, Nk1, Nk2, NP,
+!               topsurface_atom_index
 !   cards       LATTICE, ATOM_POSITIONS, PROJECTORS, SURFACE, KPATH_SLAB,
 !               KPLANE_SLAB, SURFACE_ATOMS
-! The SURFACE card must keep the crystal's own (001) cell (R1 = a1, R2 = a2,
-! the default); other surfaces need the cell remap of get_hmnr_cell
-! (src/readHmnR.f90:578-766), which this program does not carry.
-module wt_surf_input
+! The SURFACE card defines the surface cell: R1', R2', R3' = U (a1, a2, a3) with
+! U settled as the reference does (third row 0 0 1 unless given, handedness
+! fix, search for R3' when the first two rows do not span a cell of the
+! crystal's volume), and H(R) is remapped onto that cell (get_hmnr_cell,
+! src/readHmnR.f90:578-766) before anything reaches the device.
+
+
+!> The surface cell of the SURFACE card and H(R) remapped onto it (host-side
+!> geometry, once per model): U with the reference's handedness fix and its
+!> search for R3' (src/readinput.f90:1575-1668, 4247-4463), the atoms in the new
+!> cell shifted so the top-surface atom sits at 0.95 along R3'
+!> (src/readinput.f90:1749-1781), and get_hmnr_cell (src/readHmnR.f90:578-766).
+module wt_surf_cell
    use para
    implicit none
    private
-   public :: read_wt_in, read_hr
+   public :: surface_umatrix, surface_cell_positions, remap_hr, inv3, cart
+
+   integer, parameter :: max_ir = 8, search_rmax = 6
+
+contains
+
+   !> Inverse of a 3 x 3 matrix (cofactors).
+   function inv3(a) result(b)
+      real(dp), intent(in) :: a(3, 3)
+      real(dp) :: b(3, 3), det
+      b(1, 1) = a(2, 2)*a(3, 3) - a(2, 3)*a(3, 2)
+      b(1, 2) = a(1, 3)*a(3, 2) - a(1, 2)*a(3, 3)
+      b(1, 3) = a(1, 2)*a(2, 3) - a(1, 3)*a(2, 2)
+      b(2, 1) = a(2, 3)*a(3, 1) - a(2, 1)*a(3, 3)
+      b(2, 2) = a(1, 1)*a(3, 3) - a(1, 3)*a(3, 1)
+      b(2, 3) = a(1, 3)*a(2, 1) - a(1, 1)*a(2, 3)
+      b(3, 1) = a(2, 1)*a(3, 2) - a(2, 2)*a(3, 1)
+      b(3, 2) = a(1, 2)*a(3, 1) - a(1, 1)*a(3, 2)
+      b(3, 3) = a(1, 1)*a(2, 2) - a(1, 2)*a(2, 1)
+      det = a(1, 1)*b(1, 1) + a(1, 2)*b(2, 1) + a(1, 3)*b(3, 1)
+      b = b/det
+   end function inv3
+
+   !> R1 . (R2 x R3)
+   real(dp) function triple(r1, r2, r3)
+      real(dp), intent(in) :: r1(3), r2(3), r3(3)
+      triple = r1(1)*(r2(2)*r3(3) - r2(3)*r3(2)) + r1(2)*(r2(3)*r3(1) - r2(1)*r3(3)) &
+         + r1(3)*(r2(1)*r3(2) - r2(2)*r3(1))
+   end function triple
+
+   !> Cartesian vector of lattice coordinates v; lat(:, i) = a_i.
+   function cart(lat, v) result(r)
+      real(dp), intent(in) :: lat(3, 3), v(3)
+      real(dp) :: r(3)
+      r = v(1)*lat(:, 1) + v(2)*lat(:, 2) + v(3)*lat(:, 3)
+   end function cart
+
+   !> U from the SURFACE rows (third row 0 0 1 unless given), as
+   !> src/readinput.f90:1618-1668 settles it.
+   subroutine surface_umatrix(lat, u)
+      real(dp), intent(in) :: lat(3, 3)
+      real(dp), intent(inout) :: u(3, 3)
+      real(dp) :: vol0, vol2
+      vol0 = triple(lat(:, 1), lat(:, 2), lat(:, 3))
+      vol2 = triple(cart(lat, u(1, :)), cart(lat, u(2, :)), cart(lat, u(3, :)))
+      if (vol2 < 0) u(3, :) = -u(3, :)
+      if (abs(abs(vol2) - abs(vol0)) > 0.001d0) call find_third_vector(lat, u, vol0)
+   end subroutine surface_umatrix
+
+   !> Rows 1 and 2 replaced by the shortest lattice vectors along them, row 3 the
+   !> shortest lattice vector giving the smallest non-zero volume
+   !> (src/readinput.f90:4247-4463).
+   subroutine find_third_vector(lat, u, vol0)
+      real(dp), intent(in) :: lat(3, 3), vol0
+      real(dp), intent(inout) :: u(3, 3)
+      integer :: i1, i2, i3, row
+      real(dp) :: v(3), cr(3), best, smallest, shortest, vol, len3, r1(3), r2(3), urow(3)
+      logical :: done
+      do row = 1, 2
+         best = 9999999d0
+         urow = u(row, :)
+         do i1 = -search_rmax, search_rmax
+            do i2 = -search_rmax, search_rmax
+               do i3 = -search_rmax, search_rmax
+                  if (i1 == 0 .and. i2 == 0 .and. i3 == 0) cycle
+                  v = (/dble(i1), dble(i2), dble(i3)/)
+                  cr(1) = urow(2)*v(3) - v(2)*urow(3)
+                  cr(2) = urow(3)*v(1) - v(3)*urow(1)
+                  cr(3) = urow(1)*v(2) - v(1)*urow(2)
+                  if (sum(abs(cr)) < eps9 .and. sum(v*urow) > 0) then
+                     len3 = sqrt(sum(cart(lat, v)**2))
+                     if (len3 < best) then
+                        best = len3
+                        u(row, :) = v
+                     end if
+                  end if
+               end do
+            end do
+         end do
+      end do
+      r1 = cart(lat, u(1, :))
+      r2 = cart(lat, u(2, :))
+      smallest = 9999999d0
+      shortest = 9999999d0
+      do i1 = -search_rmax, search_rmax
+         do i2 = -search_rmax, search_rmax
+            do i3 = -search_rmax, search_rmax
+               if (i1 == 0 .and. i2 == 0 .and. i3 == 0) cycle
+               vol = abs(triple(r1, r2, cart(lat, (/dble(i1), dble(i2), dble(i3)/))))
+               if (vol < eps9) cycle
+               if (vol < smallest) smallest = vol
+            end do
+         end do
+      end do
+      do i1 = -search_rmax, search_rmax
+         do i2 = -search_rmax, search_rmax
+            do i3 = -search_rmax, search_rmax
+               if (i1 == 0 .and. i2 == 0 .and. i3 == 0) cycle
+               v = (/dble(i1), dble(i2), dble(i3)/)
+               vol = abs(triple(r1, r2, cart(lat, v)))
+               if (abs(vol - smallest) < eps9) shortest = min(shortest, sqrt(sum(cart(lat, v)**2)))
+            end do
+         end do
+      end do
+      done = .false.
+      do i1 = -search_rmax, search_rmax
+         do i2 = -search_rmax, search_rmax
+            do i3 = -search_rmax, search_rmax
+               if (done) exit
+               if (i1 == 0 .and. i2 == 0 .and. i3 == 0) cycle
+               v = (/dble(i1), dble(i2), dble(i3)/)
+               vol = abs(triple(r1, r2, cart(lat, v)))
+               if (abs(vol - smallest) < eps9 .and. abs(sqrt(sum(cart(lat, v)**2)) - shortest) < eps9) then
+                  u(3, :) = v
+                  done = .true.
+               end if
+            end do
+         end do
+      end do
+      if (triple(cart(lat, u(1, :)), cart(lat, u(2, :)), cart(lat, u(3, :))) < 0) u(3, :) = -u(3, :)
+      if (abs(triple(cart(lat, u(1, :)), cart(lat, u(2, :)), cart(lat, u(3, :))) - vol0) > eps9) then
+         write(*, '(a)') ' >>> ERROR: SURFACE card: no cell of the crystal''s volume spans the first two'
+         write(*, '(a)') '     vectors; the Miller-index construction is not supported here'
+         stop 1
+      end if
+   end subroutine find_third_vector
+
+   !> Each component into [0, 1) (src/readinput.f90:4495-4522).
+   subroutine home(x)
+      real(dp), intent(inout) :: x(3)
+      integer :: i
+      do i = 1, 3
+         do while (.not. (x(i) >= 0d0 .and. x(i) < 1d0))
+            if (x(i) < 0d0) then
+               x(i) = x(i) + 1d0
+            else
+               x(i) = x(i) - 1d0
+            end if
+         end do
+      end do
+   end subroutine home
+
+   !> Fractional positions in the new cell; top is the 1-based top-surface atom
+   !> (0: the largest third coordinate, first of equals).
+   subroutine surface_cell_positions(u, pos, top, posn)
+      real(dp), intent(in) :: u(3, 3), pos(:, :)
+      integer, intent(in) :: top
+      real(dp), intent(out) :: posn(:, :)
+      real(dp) :: ui(3, 3), shift(3)
+      integer :: ia, it
+      ui = inv3(u)
+      do ia = 1, size(pos, 2)
+         posn(:, ia) = ui(1, :)*pos(1, ia) + ui(2, :)*pos(2, ia) + ui(3, :)*pos(3, ia)
+         call home(posn(:, ia))
+      end do
+      it = top
+      if (it == 0) then
+         it = 1
+         do ia = 2, size(pos, 2)
+            if (posn(3, ia) > posn(3, it)) it = ia
+         end do
+      end if
+      shift = (/0d0, 0d0, 0.95d0 - posn(3, it)/)
+      do ia = 1, size(pos, 2)
+         posn(:, ia) = posn(:, ia) + shift
+         call home(posn(:, ia))
+      end do
+   end subroutine surface_cell_positions
+
+   !> get_hmnr_cell: <i,0|H|j,R> / ndegen(R) moves to
+   !> R' = U^-T (R + tau_j - tau_i) - (tau'_j - tau'_i), rounded; dropped beyond
+   !> max_ir; R' listed with the first component fastest; a later hopping on an
+   !> occupied (R', i, j) replaces the earlier one.
+   subroutine remap_hr(u, atom_of, pos, posn, nr, irvec, hmnr, ndeg)
+      real(dp), intent(in) :: u(3, 3), pos(:, :), posn(:, :)
+      integer, intent(in) :: atom_of(:), nr, irvec(:, :), ndeg(:)
+      complex(dp), intent(in) :: hmnr(:, :, :)
+      integer, allocatable :: rp(:, :, :, :), map(:, :, :)
+      logical, allocatable :: used(:, :, :)
+      real(dp) :: ui(3, 3), rmn(3), rnew(3)
+      integer :: ir, i, j, n, a, b, c, nw
+      nw = size(hmnr, 1)
+      ui = inv3(u)
+      allocate(rp(3, nw, nw, nr), map(-max_ir:max_ir, -max_ir:max_ir, -max_ir:max_ir))
+      allocate(used(-max_ir:max_ir, -max_ir:max_ir, -max_ir:max_ir))
+      used = .false.
+      do ir = 1, nr
+         do j = 1, nw
+            do i = 1, nw
+               rmn = irvec(:, ir) + pos(:, atom_of(j)) - pos(:, atom_of(i))
+               rnew = ui(1, :)*rmn(1) + ui(2, :)*rmn(2) + ui(3, :)*rmn(3) &
+                  - (posn(:, atom_of(j)) - posn(:, atom_of(i)))
+               rp(:, i, j, ir) = nint(rnew)
+               if (any(abs(rp(:, i, j, ir)) > max_ir)) cycle
+               used(rp(1, i, j, ir), rp(2, i, j, ir), rp(3, i, j, ir)) = .true.
+            end do
+         end do
+      end do
+      nrpts_surfacecell = count(used)
+      if (allocated(irvec_surfacecell)) deallocate(irvec_surfacecell)
+      if (allocated(HmnR_surfacecell)) deallocate(HmnR_surfacecell)
+      if (allocated(ndegen_surfacecell)) deallocate(ndegen_surfacecell)
+      allocate(irvec_surfacecell(3, nrpts_surfacecell))
+      allocate(HmnR_surfacecell(nw, nw, nrpts_surfacecell), ndegen_surfacecell(nrpts_surfacecell))
+      n = 0
+      map = 0
+      do c = -max_ir, max_ir
+         do b = -max_ir, max_ir
+            do a = -max_ir, max_ir
+               if (.not. used(a, b, c)) cycle
+               n = n + 1
+               irvec_surfacecell(:, n) = (/a, b, c/)
+               map(a, b, c) = n
+            end do
+         end do
+      end do
+      HmnR_surfacecell = (0d0, 0d0)
+      ndegen_surfacecell = 1
+      do ir = 1, nr
+         do j = 1, nw
+            do i = 1, nw
+               if (any(abs(rp(:, i, j, ir)) > max_ir)) cycle
+               n = map(rp(1, i, j, ir), rp(2, i, j, ir), rp(3, i, j, ir))
+               HmnR_surfacecell(i, j, n) = hmnr(i, j, ir)/ndeg(ir)
+            end do
+         end do
+      end do
+   end subroutine remap_hr
+
+end module wt_surf_cell
+
+
+module wt_surf_input
+   use para
+   use wt_surf_cell
+   implicit none
+   private
+   public :: read_wt_in, read_hr, dump_cell
+
+   ! the surface cell for read_hr's remap: U, the atoms' fractional positions
+   ! in the crystal's and in the new cell, and the atom of every orbital
+   real(dp) :: cell_u(3, 3)
+   real(dp), allocatable :: cell_pos(:, :), cell_posn(:, :)
+   integer, allocatable :: orb_atom(:)
 
    integer, parameter :: maxkv = 512
    character(len=64) :: keys(maxkv)
@@ -248,11 +501,14 @@ contains
       character(len=*), intent(in) :: fname
       real(dp) :: eta_arc, e_arc, lat(3, 3), r1(3), r2(3), s(3, 3), ra2(2), rb2(2)
       real(dp) :: vol, kstart(2), kend(2), kp(2, 64), ke(2, 64), k1(2), k2(2), t1, temp
+      real(dp) :: row3(3), linv(3, 3)
       integer :: u, ios, i, j, ia, io, natoms, nk2lines, ntop_atoms, nbot_atoms, nproj_tot
+      integer :: top_atom
       integer, allocatable :: nprojs(:), orbitals_start(:), topatoms(:), botatoms(:)
       character(len=256) :: line
       character(len=16) :: name1, name2
-      logical :: have_surface
+      character(len=64) :: word
+      logical :: have_surface, angstrom
 
       call scan_namelists(fname)
       call get_string('Hrfile', Hrfile)
@@ -280,6 +536,8 @@ contains
       call get_int('Nk1', Nk1)
       call get_int('Nk2', Nk2)
       call get_int('NP', Np)
+      top_atom = 0
+      call get_int('topsurface_atom_index', top_atom)
       if (eta_arc > -998d0) Fermi_broadening = eta_arc
       if (e_arc > -998d0) iso_energy = e_arc
       iso_energy = iso_energy*eV2Hartree
@@ -290,14 +548,27 @@ contains
       open(newunit=u, file=fname, status='old', action='read')
       ! LATTICE: unit line, then a1, a2, a3 as rows
       if (.not. seek_card(u, 'LATTICE')) stop ' >>> ERROR: wt.in has no LATTICE card'
-      read(u, '(a)') line
+      read(u, *) word
       do i = 1, 3
          read(u, *) lat(:, i)
       end do
-      if (index(lower(line), 'bohr') == 0) lat = lat*Angstrom2atomic
-      ! ATOM_POSITIONS / PROJECTORS: orbitals per atom
+      ! the unit line counts as Angstrom when it says so, Bohr otherwise (src/readinput.f90:772-787)
+      angstrom = index(lower(word), 'ang') > 0
+      if (angstrom) lat = lat*Angstrom2atomic
+      ! ATOM_POSITIONS: fractional, or Cartesian in the lattice's unit; the "D"
+      ! test is case-sensitive as in src/readinput.f90:871-933
       if (.not. seek_card(u, 'ATOM_POSITIONS')) stop ' >>> ERROR: wt.in has no ATOM_POSITIONS card'
       read(u, *) natoms
+      read(u, *) word
+      allocate(cell_pos(3, natoms), cell_posn(3, natoms))
+      linv = inv3(lat)
+      do ia = 1, natoms
+         read(u, *) name1, cell_pos(:, ia)
+         if (index(word, 'D') == 0) then
+            if (angstrom) cell_pos(:, ia) = cell_pos(:, ia)*Angstrom2atomic
+            cell_pos(:, ia) = matmul(linv, cell_pos(:, ia))
+         end if
+      end do
       if (.not. seek_card(u, 'PROJECTORS')) stop ' >>> ERROR: wt.in has no PROJECTORS card'
       allocate(nprojs(natoms), orbitals_start(natoms + 1))
       read(u, *) nprojs
@@ -309,8 +580,16 @@ contains
       Num_wann = nproj_tot
       if (Soc > 0) Num_wann = 2*nproj_tot
       Ndim = Num_wann*Np
+      ! atom of every orbital, spin down repeating spin up (spinorbital_to_atom_index)
+      allocate(orb_atom(Num_wann))
+      do ia = 1, natoms
+         do j = 1, nprojs(ia)
+            orb_atom(orbitals_start(ia) + j - 1) = ia
+            if (Soc > 0) orb_atom(orbitals_start(ia) + j - 1 + nproj_tot) = ia
+         end do
+      end do
 
-      ! SURFACE: R1, R2 in lattice-vector units; only the crystal's own cell
+      ! SURFACE: R1', R2' (and optionally R3') in units of a1, a2, a3
       s = 0d0
       s(1, 1) = 1d0
       s(2, 2) = 1d0
@@ -319,13 +598,14 @@ contains
       if (have_surface) then
          read(u, *) s(1, :)
          read(u, *) s(2, :)
+         read(u, *, iostat=ios) row3
+         if (ios == 0) s(3, :) = row3
       end if
-      if (any(abs(s(1, :) - (/1d0, 0d0, 0d0/)) > 1d-9) .or. any(abs(s(2, :) - (/0d0, 1d0, 0d0/)) > 1d-9)) then
-         write(*, '(a)') ' >>> ERROR: this build supports SURFACE = (1 0 0; 0 1 0) only'
-         stop 1
-      end if
-      r1 = lat(:, 1)
-      r2 = lat(:, 2)
+      call surface_umatrix(lat, s)
+      cell_u = s
+      call surface_cell_positions(cell_u, cell_pos, top_atom, cell_posn)
+      r1 = cart(lat, s(1, :))
+      r2 = cart(lat, s(2, :))
       ! surface frame and reciprocal vectors (src/readinput.f90:1812-1855)
       Urot = 0d0
       Urot(1, :) = r1/sqrt(sum(r1**2))
@@ -430,12 +710,13 @@ contains
    end subroutine read_wt_in
 
    !> Wannier90 *_hr.dat (src/readHmnR.f90:3-305 for the orthogonal basis):
-   !> E_fermi off the R = 0 diagonal, eV -> Hartree; with the crystal's own
-   !> surface cell, HmnR_surfacecell = HmnR and ndegen is applied on use.
+   !> E_fermi off the R = 0 diagonal, eV -> Hartree, then remapped onto the
+   !> surface cell (get_hmnr_cell, with 1/ndegen applied there).
    subroutine read_hr()
       integer :: u, ios, nw, nr, ir, m, n, i, r(3), mm, nn
       real(dp) :: re, im
-      integer, allocatable :: ndeg(:)
+      integer, allocatable :: ndeg(:), irv(:, :)
+      complex(dp), allocatable :: hmnr(:, :, :)
       character(len=256) :: line
       open(newunit=u, file=trim(Hrfile), status='old', action='read', iostat=ios)
       if (ios /= 0) then
@@ -449,35 +730,59 @@ contains
          write(*, '(a, i0, a, i0)') ' >>> ERROR: Hrfile num_wann ', nw, ' but PROJECTORS give ', Num_wann
          stop 1
       end if
-      allocate(ndeg(nr))
+      allocate(ndeg(nr), irv(3, nr), hmnr(nw, nw, nr))
       read(u, *) ndeg
-      nrpts_surfacecell = nr
-      allocate(HmnR_surfacecell(nw, nw, nr), irvec_surfacecell(3, nr), ndegen_surfacecell(nr))
-      HmnR_surfacecell = (0d0, 0d0)
+      hmnr = (0d0, 0d0)
       do ir = 1, nr
          do n = 1, nw
             do m = 1, nw
                read(u, *) r, mm, nn, re, im
-               HmnR_surfacecell(mm, nn, ir) = dcmplx(re, im)
+               hmnr(mm, nn, ir) = dcmplx(re, im)
             end do
          end do
-         irvec_surfacecell(:, ir) = r
+         irv(:, ir) = r
       end do
       close(u)
       do ir = 1, nr
-         if (all(irvec_surfacecell(:, ir) == 0)) then
+         if (all(irv(:, ir) == 0)) then
             do i = 1, nw
-               HmnR_surfacecell(i, i, ir) = HmnR_surfacecell(i, i, ir) - E_fermi
+               hmnr(i, i, ir) = hmnr(i, i, ir) - E_fermi
             end do
          end if
       end do
       if (index(Particle, 'phonon') /= 0) then
-         HmnR_surfacecell = HmnR_surfacecell*eV2Hartree*eV2Hartree
+         hmnr = hmnr*eV2Hartree*eV2Hartree
       else
-         HmnR_surfacecell = HmnR_surfacecell*eV2Hartree
+         hmnr = hmnr*eV2Hartree
       end if
-      ndegen_surfacecell = ndeg
+      call remap_hr(cell_u, orb_atom, cell_pos, cell_posn, nr, irv, hmnr, ndeg)
    end subroutine read_hr
+
+   !> Urot, Ka2, Kb2, TopOrbitals, BottomOrbitals and HmnR_surfacecell (Hartree,
+   !> ndegen applied) as plain text, full precision.
+   subroutine dump_cell(fname)
+      character(len=*), intent(in) :: fname
+      integer :: u, ir, i, j
+      open(newunit=u, file=fname)
+      write(u, '(a)') '# Urot rows, Ka2, Kb2, Num_wann Np nrpts, top, bottom, then R1 R2 R3 i j Re Im'
+      do i = 1, 3
+         write(u, '(3es25.16)') Urot(i, :)
+      end do
+      write(u, '(2es25.16)') Ka2
+      write(u, '(2es25.16)') Kb2
+      write(u, '(3i8)') Num_wann, Np, nrpts_surfacecell
+      write(u, '(*(i6))') TopOrbitals
+      write(u, '(*(i6))') BottomOrbitals
+      do ir = 1, nrpts_surfacecell
+         do j = 1, Num_wann
+            do i = 1, Num_wann
+               write(u, '(5i6, 2es25.16)') irvec_surfacecell(:, ir), i, j, &
+                  HmnR_surfacecell(i, j, ir)/ndegen_surfacecell(ir)
+            end do
+         end do
+      end do
+      close(u)
+   end subroutine dump_cell
 
 end module wt_surf_input
 
@@ -502,10 +807,25 @@ program wt_surf
    use wtsurf_gpu
    implicit none
    integer :: c0, c1, rate
+   character(len=64) :: arg
    open(unit=stdout, file='WT.out')
    write(stdout, '(a)') ' wt_surf: WannierTools surface Green''s function path on the GPU (libwtsurf)'
    call read_wt_in('wt.in')
    call read_hr()
+   ! `wt_surf --dump-cell`: write the surface cell and the remapped H(R) to
+   ! surface_cell.dat and run no driver (no device work)
+   if (command_argument_count() >= 1) then
+      call get_command_argument(1, arg)
+      if (trim(arg) == '--dump-cell') then
+         call dump_cell('surface_cell.dat')
+         SlabSS_calc = .false.
+         SlabQPI_kpath_calc = .false.
+         SlabArc_calc = .false.
+         SlabQPI_calc = .false.
+         SlabQPI_kplane_calc = .false.
+         SlabSpintexture_calc = .false.
+      end if
+   end if
    write(stdout, '(a, i0, a, i0, a, i0)') ' Num_wann ', Num_wann, '  Np ', Np, '  Ndim ', Ndim
    if (SlabSS_calc) then
       write(stdout, *) ' '
