@@ -290,22 +290,44 @@ __device__ __forceinline__ bool gj_inverse(unsigned char *sbuf, unsigned char *t
     C(*S)[LD] = reinterpret_cast<C(*)[LD]>(sbuf);
     C(*T)[NP] = reinterpret_cast<C(*)[NP]>(tbuf);
 
-    for (int e = tid; e < NP * NP; e += NT) {
-        const int i = e % NP, j = e / NP;
-        C v;
-        v.x = 0;
-        v.y = 0;
-        if (i < n && j < n) {
-            double2 x = E[i + j * NP];
-            if (E2) {
-                const double2 y = E2[i + j * NP], h = H[i + j * NP];
-                x.x = (x.x + y.x) - h.x;
-                x.y = (x.y + y.y) - h.y;
+    // Load in batches of kLoadBatch elements per thread, every global load of
+    // a batch issued before its first LDS store: S is reached through a
+    // generic pointer, so the compiler would otherwise keep each load behind
+    // the previous store and pay one memory round trip per element.
+    constexpr int kIter = (NP * NP + NT - 1) / NT;
+    constexpr int kLoadBatch = 5;
+    for (int it0 = 0; it0 < kIter; it0 += kLoadBatch) {
+        double2 xv[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int e = tid + (it0 + u) * NT;
+            const int i = e % NP, j = e / NP;
+            xv[u] = make_double2(0.0, 0.0);
+            if (it0 + u < kIter && e < NP * NP && i < n && j < n) {
+                double2 x = E[i + j * NP];
+                if (E2) {
+                    const double2 y = E2[i + j * NP], h = H[i + j * NP];
+                    x.x = (x.x + y.x) - h.x;
+                    x.y = (x.y + y.y) - h.y;
+                }
+                xv[u] = x;
             }
-            v.x = (R)((i == j ? z.x : 0.0) - x.x);
-            v.y = (R)((i == j ? z.y : 0.0) - x.y);
         }
-        S[j][i] = v;
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const int e = tid + (it0 + u) * NT;
+            const int i = e % NP, j = e / NP;
+            if (it0 + u < kIter && e < NP * NP) {
+                C v;
+                v.x = 0;
+                v.y = 0;
+                if (i < n && j < n) {
+                    v.x = (R)((i == j ? z.x : 0.0) - xv[u].x);
+                    v.y = (R)((i == j ? z.y : 0.0) - xv[u].y);
+                }
+                S[j][i] = v;
+            }
+        }
     }
     for (int i = tid; i < NP; i += NT) cm.pivstep[i] = -1;
     if (tid == 0) cm.sing = 0;
@@ -924,21 +946,39 @@ __device__ __forceinline__ void gj_round(const DecimateArgs &a, int slot, const 
         const double2 *H00 = a.hk + (size_t)ik * 2 * NN;
         const double2 *H01 = H00 + NN;
         C *al = w.alpha(0), *be = w.beta(0);
-        for (int e = tid; e < NN; e += NT) {
-            const double2 h = H00[e];
-            w.es[e] = h;
-            w.et[e] = h;
-            const double2 h1 = H01[e];
-            C av;
-            av.x = (R)h1.x;
-            av.y = (R)h1.y;
-            al[e] = av;
-            const int i = e % NP, j = e / NP;
-            const double2 ht = H01[j + i * NP];
-            C bv;
-            bv.x = (R)ht.x;
-            bv.y = (R)(-ht.y);
-            be[e] = bv;
+        // batches whose loads are all issued before their stores (the stores
+        // may alias the loads as far as the compiler knows)
+        constexpr int kIter = (NN + NT - 1) / NT, kBatch = 4;
+        for (int it0 = 0; it0 < kIter; it0 += kBatch) {
+            double2 h[kBatch], h1[kBatch], ht[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) h[u] = h1[u] = ht[u] = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = tid + (it0 + u) * NT;
+                if (it0 + u < kIter && e < NN) {
+                    const int i = e % NP, j = e / NP;
+                    h[u] = H00[e];
+                    h1[u] = H01[e];
+                    ht[u] = H01[j + i * NP];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int e = tid + (it0 + u) * NT;
+                if (it0 + u < kIter && e < NN) {
+                    w.es[e] = h[u];
+                    w.et[e] = h[u];
+                    C av;
+                    av.x = (R)h1[u].x;
+                    av.y = (R)h1[u].y;
+                    al[e] = av;
+                    C bv;
+                    bv.x = (R)ht[u].x;
+                    bv.y = (R)(-ht[u].y);
+                    be[e] = bv;
+                }
+            }
         }
         if (tid == 0) {
             cm.s.it = 1;
