@@ -1248,7 +1248,9 @@ static int launch_phase(const DecimateArgs &a, int q, int precision, int per_set
 }
 
 int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const RoundCtl &ctl) {
-    hipStream_t s0 = ctl.s[0], s1 = ctl.s[1], sc = ctl.s[2];
+    constexpr int NS = kSlotSets;
+    const hipStream_t s0 = ctl.s[0], sc = ctl.s[NS];
+    const hipEvent_t start = ctl.ev[2 * NS + 2];
     if (precision == 0)
         dispatch_allow_lds<double>(q);
     else
@@ -1257,45 +1259,50 @@ int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const
     // inverse rounds (start, three finals); a slot runs ceil(ninst / slots)
     // instances.  Past that bound (with slack) the device has stopped making
     // progress and the run fails instead of launching rounds forever.
-    const long per_slot = (a.ninst + 2L * per_set - 1) / (2L * per_set) + 1;
+    const long slots = (long)NS * per_set;
+    const long per_slot = (a.ninst + slots - 1) / slots + 1;
     const long max_batches = per_slot * (a.itermax + 4) / kRoundsPerBatch + 4;
-    if (hipMemsetAsync(a.state, 0, (size_t)2 * per_set * sizeof(SlotState), s0) != hipSuccess ||
+    if (hipMemsetAsync(a.state, 0, (size_t)slots * sizeof(SlotState), s0) != hipSuccess ||
         hipMemsetAsync(a.queue, 0, 2 * sizeof(unsigned), s0) != hipSuccess ||
-        hipEventRecord(ctl.ev[6], s0) != hipSuccess || hipStreamWaitEvent(s1, ctl.ev[6], 0) != hipSuccess)
+        hipEventRecord(start, s0) != hipSuccess)
         return -1;
-    // Stream 0 runs set 0 as (inverse, GEMMs) rounds, stream 1 runs set 1 as
-    // (GEMMs, inverse): the sets stay half a round apart.  Each stream marks
-    // the end of every batch of rounds with an event; a third stream waits for
-    // both marks of a batch and copies the finished-instance count to pinned
-    // host memory, so the count it reports covers whole batches of both sets
-    // whichever stream runs ahead, and the two work streams never wait for
-    // each other.  The host reads the count of the batch before while this one
-    // runs, so the device never waits for the host.
+    for (int k = 1; k < NS; ++k)
+        if (hipStreamWaitEvent(ctl.s[k], start, 0) != hipSuccess) return -1;
+    // Stream k runs set k as (inverse, GEMMs) rounds for even k and (GEMMs,
+    // inverse) for odd k: the sets stay half a round apart.  Each stream marks
+    // the end of every batch of rounds with an event; the counter stream waits
+    // for every set's mark of a batch and copies the finished-instance count
+    // to pinned host memory, so the count it reports covers whole batches of
+    // all sets whichever stream runs ahead, and the work streams never wait
+    // for each other.  The host reads the count of the batch before while
+    // this one runs, so the device never waits for the host.
     for (long b = 0;; ++b) {
         if (b > max_batches) return -2;
-        for (int r = 0; r < kRoundsPerBatch; ++r) {
-            if (launch_phase(a, q, precision, per_set, 0, true, s0) ||
-                launch_phase(a, q, precision, per_set, 0, false, s0) ||
-                launch_phase(a, q, precision, per_set, 1, false, s1) ||
-                launch_phase(a, q, precision, per_set, 1, true, s1))
-                return -1;
-        }
+        for (int r = 0; r < kRoundsPerBatch; ++r)
+            for (int k = 0; k < NS; ++k) {
+                const bool gj_first = (k & 1) == 0;
+                if (launch_phase(a, q, precision, per_set, k, gj_first, ctl.s[k]) ||
+                    launch_phase(a, q, precision, per_set, k, !gj_first, ctl.s[k]))
+                    return -1;
+            }
         const int h = (int)(b & 1);
-        if (hipEventRecord(ctl.ev[h], s0) != hipSuccess ||
-            hipEventRecord(ctl.ev[2 + h], s1) != hipSuccess ||
-            hipStreamWaitEvent(sc, ctl.ev[h], 0) != hipSuccess ||
-            hipStreamWaitEvent(sc, ctl.ev[2 + h], 0) != hipSuccess ||
-            hipMemcpyAsync(ctl.pinned + h, a.queue + 1, sizeof(unsigned), hipMemcpyDeviceToHost, sc) !=
+        for (int k = 0; k < NS; ++k)
+            if (hipEventRecord(ctl.ev[2 * k + h], ctl.s[k]) != hipSuccess ||
+                hipStreamWaitEvent(sc, ctl.ev[2 * k + h], 0) != hipSuccess)
+                return -1;
+        if (hipMemcpyAsync(ctl.pinned + h, a.queue + 1, sizeof(unsigned), hipMemcpyDeviceToHost, sc) !=
                 hipSuccess ||
-            hipEventRecord(ctl.ev[4 + h], sc) != hipSuccess)
+            hipEventRecord(ctl.ev[2 * NS + h], sc) != hipSuccess)
             return -1;
         if (b > 0) {
-            if (hipEventSynchronize(ctl.ev[4 + (h ^ 1)]) != hipSuccess) return -1;
+            if (hipEventSynchronize(ctl.ev[2 * NS + (h ^ 1)]) != hipSuccess) return -1;
             if (ctl.pinned[h ^ 1] >= (unsigned)a.ninst) break;
         }
     }
-    if (hipEventRecord(ctl.ev[6], s1) != hipSuccess || hipStreamWaitEvent(s0, ctl.ev[6], 0) != hipSuccess)
-        return -1;
+    for (int k = 1; k < NS; ++k)
+        if (hipEventRecord(ctl.ev[2 * k], ctl.s[k]) != hipSuccess ||
+            hipStreamWaitEvent(s0, ctl.ev[2 * k], 0) != hipSuccess)
+            return -1;
     return 0;
 }
 

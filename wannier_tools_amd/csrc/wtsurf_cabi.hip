@@ -48,10 +48,10 @@ struct wtsurf_ctx {
     int num_cu = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    // round driver of the decimation: a second stream, its events and a
-    // pinned progress counter
-    hipStream_t aux = nullptr, cnt = nullptr;
-    hipEvent_t drv[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    // round driver of the decimation: the other sets' work streams, the
+    // counter stream, their events and a pinned progress counter
+    hipStream_t aux[kSlotSets - 1] = {}, cnt = nullptr;
+    hipEvent_t drv[kRoundEvents] = {};
     unsigned *pinned = nullptr;
     std::string err;
 
@@ -136,12 +136,12 @@ static int ensure_work(wtsurf_ctx *c, int q, int precision, int ninst) {
     const size_t per_slot = decimate_slot_bytes(q, precision);
     if (per_slot == 0) return c->fail("no decimation kernel for this order");
     const int per_set = decimate_slots_per_set(ninst, c->num_cu);
-    const size_t want = (size_t)2 * per_set * per_slot;
+    const size_t want = (size_t)kSlotSets * per_set * per_slot;
     if (want > c->work.n) {
         c->work.release();
         if (c->work.ensure(want)) return c->fail("out of device memory (decimation workspace)");
     }
-    if (c->state.ensure((size_t)2 * per_set * sizeof(SlotState)) || c->queue.ensure(2))
+    if (c->state.ensure((size_t)kSlotSets * per_set * sizeof(SlotState)) || c->queue.ensure(2))
         return c->fail("out of device memory (slot state)");
     c->per_set = per_set;
     return 0;
@@ -160,8 +160,9 @@ int wtsurf_ctx_create(int device, wtsurf_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return -2; }
     c->num_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+    bool aux_ok = true;
+    for (auto &a : c->aux) aux_ok = aux_ok && hipStreamCreateWithFlags(&a, hipStreamNonBlocking) == hipSuccess;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess || !aux_ok ||
         hipStreamCreateWithFlags(&c->cnt, hipStreamNonBlocking) != hipSuccess ||
         hipHostMalloc((void **)&c->pinned, 2 * sizeof(unsigned), hipHostMallocDefault) != hipSuccess) {
         wtsurf_ctx_destroy(c);
@@ -182,7 +183,8 @@ void wtsurf_ctx_destroy(wtsurf_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->aux) (void)hipStreamSynchronize(c->aux);
+    for (auto a : c->aux)
+        if (a) (void)hipStreamSynchronize(a);
     if (c->cnt) (void)hipStreamSynchronize(c->cnt);
     c->hr.release(); c->irvec.release(); c->slot_begin.release();
     c->d_top.release(); c->d_bot.release();
@@ -197,7 +199,8 @@ void wtsurf_ctx_destroy(wtsurf_ctx *c) {
         if (e) (void)hipEventDestroy(e);
     if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
-    if (c->aux) (void)hipStreamDestroy(c->aux);
+    for (auto a : c->aux)
+        if (a) (void)hipStreamDestroy(a);
     if (c->cnt) (void)hipStreamDestroy(c->cnt);
     delete c;
 }
@@ -355,12 +358,12 @@ static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const
     }
     RoundCtl ctl{};
     ctl.s[0] = c->stream;
-    ctl.s[1] = c->aux;
-    ctl.s[2] = c->cnt;
-    for (int i = 0; i < 7; ++i) ctl.ev[i] = c->drv[i];
+    for (int k = 1; k < kSlotSets; ++k) ctl.s[k] = c->aux[k - 1];
+    ctl.s[kSlotSets] = c->cnt;
+    for (int i = 0; i < kRoundEvents; ++i) ctl.ev[i] = c->drv[i];
     ctl.pinned = c->pinned;
     if (int r = decimate_run(a, q, precision, c->per_set, ctl)) {
-        (void)hipStreamSynchronize(c->aux);  // rounds already queued on the other streams
+        for (auto a : c->aux) (void)hipStreamSynchronize(a);  // rounds already queued there
         (void)hipStreamSynchronize(c->cnt);
         if (r == -2) return c->fail("decimation made no progress within its round bound");
         const hipError_t e = hipGetLastError();
@@ -461,7 +464,7 @@ int wtsurf_sync(wtsurf_ctx *c) {
     if (!c) return -1;
     if (int r = use_device(c)) return r;
     WT_HIP(c, hipStreamSynchronize(c->stream));
-    WT_HIP(c, hipStreamSynchronize(c->aux));
+    for (auto a : c->aux) WT_HIP(c, hipStreamSynchronize(a));
     return 0;
 }
 

@@ -35,8 +35,8 @@ struct DecimateArgs {
     const int *bot;
     int nbot;
     double divisor;         // 1 (surfstat) or pi (SurfaceDOSkk)
-    void *work;             // [2*per_set][decimate_slot_bytes]
-    SlotState *state;       // [2*per_set], zeroed by decimate_run
+    void *work;             // [kSlotSets*per_set][decimate_slot_bytes]
+    SlotState *state;       // [kSlotSets*per_set], zeroed by decimate_run
     unsigned *queue;        // [2]: next instance ticket, finished instances
     double *dos_l, *dos_r, *dos_bulk;  // [ninst] in (ik + nk*iw) order
     int *iters;             // [ninst]
@@ -52,12 +52,18 @@ struct DecimateArgs {
     }
 };
 
-// Streams and host-visible progress counter of the round driver: two work
-// streams and a counter-copy stream, all at default priority; seven events;
-// two pinned unsigneds.
+// Slot sets of the decimation, one work stream each: the sets' rounds
+// interleave on every CU (one Gauss-Jordan and one GEMM workgroup at a time).
+constexpr int kSlotSets = 2;
+
+// Streams and host-visible progress counter of the round driver: kSlotSets
+// work streams and a counter-copy stream, all at default priority; events
+// (two per set for the batch marks, two for the counter copies, one start
+// mark); two pinned unsigneds.
+constexpr int kRoundEvents = 2 * kSlotSets + 3;
 struct RoundCtl {
-    hipStream_t s[3];
-    hipEvent_t ev[7];
+    hipStream_t s[kSlotSets + 1];
+    hipEvent_t ev[kRoundEvents];
     unsigned *pinned;
 };
 
@@ -67,9 +73,10 @@ int decimate_max_q();       // largest Q the decimation kernels are built for
 int decimate_slots_per_set(int ninst, int num_cu);
 // precision: 0 = FP64, 1 = FP32 inverse/GEMMs with FP64 accumulation and finals
 size_t decimate_slot_bytes(int q, int precision);
-// Enqueues the decimation of a.ninst instances on ctl.s[0] and ctl.s[1] and
-// returns once every instance is counted finished (the work is complete when
-// ctl.s[0] reaches the end of what was enqueued; ctl.s[0] waits for s[1]).
+// Enqueues the decimation of a.ninst instances on the work streams
+// ctl.s[0..kSlotSets) and returns once every instance is counted finished
+// (the work is complete when ctl.s[0] reaches the end of what was enqueued;
+// ctl.s[0] waits for the others).
 // Returns -2 when the count stops short of a.ninst within the round bound.
 int decimate_run(const DecimateArgs &a, int q, int precision, int per_set, const RoundCtl &ctl);
 
