@@ -9,7 +9,9 @@ k points, energies) are resident on the device before timing starts; a step
 is the Fourier sum plus the batched decimation and traces.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32_refine|auto]
+    python bench.py --workload plane288|mixed [--steps K] [--warmup W]
     python bench.py --summarize-rocprof TRACE_DIR FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
+    python bench.py --summarize-trace TRACE_DIR KERNEL [KERNEL ...]
 
 Prints one JSON line.  The decimation runs as rounds of two kernels on two
 streams (gj_round_kernel and gemm_round_kernel); the roofline figures refer to
@@ -21,6 +23,19 @@ summary kept under profiles/.  The FP64 path always runs (it defines the per-ins
 reference iteration count I_ref used for the algorithmic flop count
 F = (56 I_ref + 24) N^3 per solve).  The reported `value` is the solves/s of
 the fastest path whose A(k, omega) error against FP64 is <= 1e-6.
+
+Two more workloads, FP64 only, print a line of the same form:
+  plane288  the 144-orbital spinful model (Ndim = 288, R in [-3,3]^3) on an
+            nk x nk sampling (default 65 x 65) of the k-plane [-0.5, 0.5]^2 at
+            omega = 0, eta = 5e-4 eV, traces divided by pi (SurfaceDOSkk);
+  mixed     four models (Num_wann, Np) = (8, 1), (16, 2), (36, 3), (72, 3)
+            (Ndim 8, 32, 108, 216; R in [-2,2]^3), 25,000 instances each with
+            k from 500 random points, omega uniform on [-1, 1] eV and eta
+            log-uniform on [1e-4, 1e-2] eV, through the per-instance batch
+            entry point (its small per-instance arrays are uploaded and the
+            traces downloaded inside each step).
+--summarize-trace prints the kernel table and stage span of a rocprofv3
+kernel trace of either.
 """
 from __future__ import annotations
 
@@ -61,15 +76,41 @@ def workload(nk: int, nomega: int):
     return irvec, hr, kpts, omegas, eta
 
 
-def time_path(gf, precision: str, steps: int, warmup: int):
+PLANE_NW, PLANE_NP = 144, 2
+PLANE_ETA_EV = 5e-4
+MIXED_MODELS = ((8, 1, 40), (16, 2, 41), (36, 3, 42), (72, 3, 43))  # (Num_wann, Np, seed)
+MIXED_PER_MODEL = 25000
+
+
+def workload_plane288(nk: int):
+    mdl = M.model_ndim288()
+    irvec, hr = mdl.surface_cell()
+    kpts, _, _ = M.kplane_slab((-0.5, -0.5), (1.0, 0.0), (0.0, 1.0), nk, nk)
+    return irvec, hr, kpts, np.array([0.0]), PLANE_ETA_EV * M.EV2HARTREE
+
+
+def workload_mixed(per_model: int):
+    rng = np.random.default_rng(4)
+    sets = []
+    for nw, np_, seed in MIXED_MODELS:
+        irvec, hr = M.synthetic_model(nw, 2, 0.5, M.decay_l2(1.5), seed=seed).surface_cell()
+        sets.append(dict(nw=nw, np=np_, irvec=irvec, hr=hr,
+                         kpts=rng.uniform(-0.5, 0.5, (500, 2)),
+                         k_index=rng.integers(0, 500, per_model).astype(np.int32),
+                         omega=rng.uniform(-1.0, 1.0, per_model) * M.EV2HARTREE,
+                         eta=10.0 ** rng.uniform(-4.0, -2.0, per_model) * M.EV2HARTREE))
+    return sets
+
+
+def time_path(gf, precision: str, steps: int, warmup: int, divisor: float = 1.0):
     for i in range(warmup):
-        gf.run_async(gf.eta, 1.0, precision)
+        gf.run_async(gf.eta, divisor, precision)
         gf.sync()
         log(f"[bench] {precision} warmup {i + 1}/{warmup}: decimation {gf.timing()[1]:.1f} ms")
     dec_ms = []
     t0 = time.perf_counter()
     for i in range(steps):
-        gf.run_async(gf.eta, 1.0, precision)
+        gf.run_async(gf.eta, divisor, precision)
         gf.sync()
         dec_ms.append(gf.timing()[1])
         log(f"[bench] {precision} step {i + 1}/{steps}: decimation {dec_ms[-1]:.1f} ms")
@@ -78,59 +119,64 @@ def time_path(gf, precision: str, steps: int, warmup: int):
     return (t1 - t0) / steps, float(np.mean(dec_ms)), out
 
 
+def alg_flops(ndim: int, iters) -> float:
+    """F = (56 I_ref + 24) N^3 summed over the instances."""
+    return float(np.sum(56.0 * np.asarray(iters, dtype=np.float64) + 24.0)) * float(ndim) ** 3
+
+
 # ---- CPU baseline: the reference's own Fortran (oracle/_ref) or the C port --
 def _cpu_worker(args):
-    kind, idx, kpts, omegas, eta, irvec, hr = args
+    """Solve the jobs (model, k1, k2, omega, eta) in order, one H00/H01 per
+    run of equal (model, k), as the reference's k-outer loops do."""
+    kind, models, jobs = args
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     import oracle
-    n_done = 0
-    if kind == "reference":
-        ref = oracle.RefHarness()
-        ref.set_model(NW, NP_, irvec, hr)
-        cache = {}
-        for ik, iw in idx:
-            if ik not in cache:
-                cache = {ik: ref.ham_qlayer2qlayer(kpts[ik])}
-            h00, h01 = cache[ik]
-            ref.surfgreen_1985(omegas[iw], eta, h00, h01)
-            n_done += 1
-    else:
-        o = oracle.Oracle()
-        cache = {}
-        for ik, iw in idx:
-            if ik not in cache:
-                cache = {ik: o.ham_qlayer2qlayer(NW, NP_, irvec, hr, kpts[ik])}
-            h00, h01 = cache[ik]
-            o.surfgreen_1985(omegas[iw], eta, h00, h01)
-            n_done += 1
-    return n_done
+    eng = oracle.RefHarness() if kind == "reference" else oracle.Oracle()
+    cur_m, key, h = None, None, None
+    for m, k1, k2, w, e in jobs:
+        nw, np_, irvec, hr = models[m]
+        if kind == "reference" and m != cur_m:
+            eng.set_model(nw, np_, irvec, hr)
+            cur_m = m
+        if (m, k1, k2) != key:
+            k = np.array([k1, k2])
+            h = eng.ham_qlayer2qlayer(k) if kind == "reference" else \
+                eng.ham_qlayer2qlayer(nw, np_, irvec, hr, k)
+            key = (m, k1, k2)
+        eng.surfgreen_1985(w, e, h[0], h[1])
+    return len(jobs)
 
 
-def cpu_baseline(kpts, omegas, eta, irvec, hr, per_worker: int, workers: int):
-    """Time the reference CPU path on a bounded, evenly strided sample of the grid:
-    `workers` processes (the reference's MPI k round-robin), one OpenBLAS thread each."""
+def cpu_baseline(models, jobs, workers: int, what: str):
+    """Time the reference CPU path on a bounded sample: `jobs` dealt
+    round-robin over `workers` processes (the reference's MPI k round-robin),
+    one OpenBLAS thread each."""
     import oracle
     kind = "reference" if oracle.ref_available() else "port"
-    nk, nom = kpts.shape[0], omegas.size
-    total = per_worker * workers
-    flat = np.linspace(0, nk * nom - 1, total).astype(np.int64)
-    pairs = [(int(t // nom), int(t % nom)) for t in flat]
-    chunks = [pairs[w::workers] for w in range(workers)]
+    chunks = [sorted(jobs[w::workers], key=lambda j: (j[0], j[1], j[2])) for w in range(workers)]
     ctx = mp.get_context("spawn")
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     with ctx.Pool(workers) as pool:
         t0 = time.perf_counter()
-        done = sum(pool.map(_cpu_worker, [(kind, c, kpts, omegas, eta, irvec, hr) for c in chunks]))
+        done = sum(pool.map(_cpu_worker, [(kind, models, c) for c in chunks]))
         dt = time.perf_counter() - t0
     return {
         "value": done / dt,
         "unit": "solves/s",
         "cores": workers,
         "kind": kind,
-        "sample": (f"{done} (k, omega) instances strided over the 512x512 grid, "
+        "sample": (f"{done} {what}, "
                    f"{'reference ham_qlayer2qlayer + surfgreen_1985 (Fortran, OpenBLAS)' if kind == 'reference' else 'C FP64 restatement'}"
                    f", {workers} processes x 1 thread, {dt:.1f} s"),
     }
+
+
+def grid_jobs(kpts, omegas, eta, total: int):
+    """`total` (k, omega) instances evenly strided over the grid."""
+    nk, nom = kpts.shape[0], omegas.size
+    flat = np.linspace(0, nk * nom - 1, total).astype(np.int64)
+    return [(0, float(kpts[t // nom, 0]), float(kpts[t // nom, 1]), float(omegas[t % nom]), eta)
+            for t in flat]
 
 
 def _csv_rows(d: str, suffix: str):
@@ -149,15 +195,36 @@ def _counter_kib(d: str, counter: str, kernels) -> float:
                if r["Counter_Name"] == counter and any(k in r["Kernel_Name"] for k in kernels))
 
 
+def _trace_table(trace, kernels, rows_shown: int = 8):
+    """Kernel table of a rocprofv3 kernel trace, and the span (first start to
+    last end) and busy time of the dispatches whose names contain `kernels`."""
+    stats = _csv_rows(trace, "kernel_stats.csv")
+    rows = _csv_rows(trace, "kernel_trace.csv")
+    sel = [r for r in rows if any(k in r["Kernel_Name"] for k in kernels)]
+    if not sel:
+        raise SystemExit(f"no dispatch of {kernels} in {trace}")
+    span_ns = max(int(r["End_Timestamp"]) for r in sel) - min(int(r["Start_Timestamp"]) for r in sel)
+    busy = {k: sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sel
+                   if k in r["Kernel_Name"]) for k in kernels}
+    print("| Kernel | Calls | Total ms | Average us |\n|---|---|---|---|")
+    for r in stats[:rows_shown]:
+        print(f"| `{r['Name'][:90]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
+              f"{float(r['AverageNs']) / 1e3:.1f} |")
+    print(f"\nSpan of the {', '.join(kernels)} dispatches (first start to last end): "
+          f"{span_ns / 1e6:.1f} ms; " + "; ".join(f"{k} busy {v / 1e6:.1f} ms" for k, v in busy.items()))
+    return span_ns
+
+
+def summarize_trace(trace, *kernels) -> int:
+    """Kernel table and span of a rocprofv3 kernel trace (any workload)."""
+    _trace_table(trace, kernels, rows_shown=12)
+    return 0
+
+
 def summarize_rocprof(trace, fetch, write, cfetch, cwrite) -> int:
     """profiles/ summary of the decimation stage from rocprofv3 runs of
     `bench.py --steps 1 --warmup 0 --no-cpu-baseline --precision fp64`."""
-    stats = _csv_rows(trace, "kernel_stats.csv")
-    rows = _csv_rows(trace, "kernel_trace.csv")
-    dec = [r for r in rows if any(k in r["Kernel_Name"] for k in DECIMATION_KERNELS)]
-    span_ns = max(int(r["End_Timestamp"]) for r in dec) - min(int(r["Start_Timestamp"]) for r in dec)
-    busy = {k: sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in dec
-                   if k in r["Kernel_Name"]) for k in DECIMATION_KERNELS}
+    span_ns = _trace_table(trace, DECIMATION_KERNELS)
     # counter scale: KiB reported for a copy of known size
     copy_bytes = 1 << 30
     f_scale = copy_bytes / (_counter_kib(cfetch, "FETCH_SIZE", ("copyBuffer",)) * 1024)
@@ -169,54 +236,15 @@ def summarize_rocprof(trace, fetch, write, cfetch, cwrite) -> int:
                "write_bytes": write_b, "fetch_scale": f_scale, "write_scale": w_scale,
                "span_ns": span_ns}, open(os.path.join(ROOT, "profiles", TRAFFIC_JSON), "w"),
               indent=1)
-    print("| Kernel | Calls | Total ms | Average us |\n|---|---|---|---|")
-    for r in stats[:8]:
-        print(f"| `{r['Name'][:90]}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.1f} | "
-              f"{float(r['AverageNs']) / 1e3:.1f} |")
-    print(f"\nDecimation stage span (first round start to last round end): {span_ns / 1e6:.1f} ms; "
-          + "; ".join(f"{k} busy {v / 1e6:.1f} ms" for k, v in busy.items()))
     print(f"\nHBM traffic of the stage: fetch {fetch_b:.4e} B (scale {f_scale:.3f}), "
           f"write {write_b:.4e} B (scale {w_scale:.3f}), {(fetch_b + write_b) / span_ns:.1f} GB/s "
           f"over the span")
     return 0
 
 
-def main() -> int:
-    if len(sys.argv) == 7 and sys.argv[1] == "--summarize-rocprof":
-        return summarize_rocprof(*sys.argv[2:])
-    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32_refine"])
-    ap.add_argument("--nk", type=int, default=512)
-    ap.add_argument("--nomega", type=int, default=512)
-    ap.add_argument("--cpu-workers", type=int, default=16)
-    ap.add_argument("--cpu-per-worker", type=int, default=400)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB MiB (counter calibration)")
-    args = ap.parse_args()
-
-    if args.calibrate_copy:
-        import torch
-        n = args.calibrate_copy * (1 << 20) // 4
-        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
-        return 0
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the surface Green's function path is not "
-              "sharded across GPUs in this build (run with --gpus 1)", file=sys.stderr)
-        return 2
-
-    from wannier_tools_amd.surface import SurfaceGF
-
-    irvec, hr, kpts, omegas, eta = workload(args.nk, args.nomega)
+def run_grid72(args, SurfaceGF) -> dict:
+    """The Ndim = 72 k-path x energy grid (the headline line)."""
+    irvec, hr, kpts, omegas, eta = workload(args.nk or 512, args.nomega or 512)
     ninst = kpts.shape[0] * omegas.size
     gf = SurfaceGF(0)
     gf.set_model_arrays(NW, NP_, irvec, hr)
@@ -226,7 +254,7 @@ def main() -> int:
     log(f"[bench] FP64 path: {ninst} instances, Ndim={NDIM}")
     s64, dec64, out64 = time_path(gf, "fp64", args.steps, args.warmup)
     dl64, dr64, db64, it64, st64 = out64
-    flops = float(np.sum(56.0 * it64.astype(np.float64) + 24.0)) * NDIM ** 3
+    flops = alg_flops(NDIM, it64)
     res = {
         "fp64": {"s_per_step": s64, "solves_per_s": ninst / s64, "decimate_ms": dec64,
                  "mean_iters": float(it64.mean()), "max_iters": int(it64.max()),
@@ -264,7 +292,7 @@ def main() -> int:
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a 1 GiB copy calibration)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", TRAFFIC_JSON)
-    if name == "fp64" and (args.nk, args.nomega) == (512, 512) and os.path.exists(tpath):
+    if name == "fp64" and (kpts.shape[0], omegas.size) == (512, 512) and os.path.exists(tpath):
         t = json.load(open(tpath))
         traffic = t["fetch_bytes"] + t["write_bytes"]
     out = {
@@ -296,8 +324,175 @@ def main() -> int:
     gf.close()
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(kpts, omegas, eta, irvec, hr, args.cpu_per_worker,
-                                           args.cpu_workers)
+        total = (args.cpu_per_worker or 400) * args.cpu_workers
+        out["cpu_baseline"] = cpu_baseline([(NW, NP_, irvec, hr)],
+                                           grid_jobs(kpts, omegas, eta, total), args.cpu_workers,
+                                           f"(k, omega) instances strided over the "
+                                           f"{kpts.shape[0]}x{omegas.size} grid")
+    return out
+
+
+def run_plane288(args, SurfaceGF) -> dict:
+    """The Ndim = 288 model on a k-plane at omega = 0 (FP64)."""
+    nk = args.nk or 65
+    irvec, hr, kpts, omegas, eta = workload_plane288(nk)
+    ninst = kpts.shape[0]
+    n = PLANE_NW * PLANE_NP
+    gf = SurfaceGF(0)
+    gf.set_model_arrays(PLANE_NW, PLANE_NP, irvec, hr)
+    gf.upload_grid(kpts, omegas)
+    gf.eta = eta
+    log(f"[bench] plane288: {ninst} instances, Ndim={n}")
+    s, dec, res = time_path(gf, "fp64", args.steps, args.warmup, M.REF_PI)
+    its, st = res[3], res[4]
+    flops = alg_flops(n, its)
+    gf.close()
+    log(f"[bench] plane288: {ninst / s:.1f} solves/s, decimation {dec:.1f} ms")
+    achieved = flops / (dec * 1e-3) / 1e12
+    out = {
+        "metric": "G(k,w) solves/s @Ndim=288, k-plane at omega=0 (FP64)",
+        "value": ninst / s, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": s * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": f"surface GF k-plane {nk}x{nk} over [-0.5,0.5]^2 at omega=0, "
+                               "144-orbital spinful synthetic model, Np=2",
+                   "ndim": n, "num_wann": PLANE_NW, "np": PLANE_NP, "nr": int(irvec.shape[0]),
+                   "nk": ninst, "instances": ninst, "eta_ev": PLANE_ETA_EV, "divisor": "pi",
+                   "stop_rule": "sum|alpha|<=1e-16 Ha, itermax 100", "path": "fp64",
+                   "mean_iters": float(its.mean()), "not_converged": int(np.count_nonzero(st & 2))},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "decimation stage: gj_large_round_kernel<18, double> + "
+                               "gemm_large_round_kernel<18, double> rounds on two streams",
+                     "flops_per_launch": flops},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline ...")
+        total = (args.cpu_per_worker or 6) * args.cpu_workers
+        out["cpu_baseline"] = cpu_baseline([(PLANE_NW, PLANE_NP, irvec, hr)],
+                                           grid_jobs(kpts, omegas, eta, total), args.cpu_workers,
+                                           f"k points strided over the {nk}x{nk} plane")
+    return out
+
+
+def run_mixed(args, SurfaceGF) -> dict:
+    """Four models, per-instance (k, omega, eta), through the batch entry point (FP64)."""
+    sets = workload_mixed(MIXED_PER_MODEL)
+    ctxs = []
+    for w in sets:
+        gf = SurfaceGF(0)
+        gf.set_model_arrays(w["nw"], w["np"], w["irvec"], w["hr"])
+        ctxs.append(gf)
+
+    def step():
+        dec, outs = [], []
+        for gf, w in zip(ctxs, sets):
+            outs.append(gf.spectral_batch(w["kpts"], w["k_index"], w["omega"], w["eta"]))
+            dec.append(gf.timing()[1])
+        return dec, outs
+
+    for i in range(args.warmup):
+        step()
+        log(f"[bench] mixed warmup {i + 1}/{args.warmup}")
+    decs = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        dec, outs = step()
+        decs.append(dec)
+        log(f"[bench] mixed step {i + 1}/{args.steps}: decimation "
+            + ", ".join(f"{d:.1f}" for d in dec) + " ms")
+    s = (time.perf_counter() - t0) / args.steps
+    for gf in ctxs:
+        gf.close()
+    ninst = MIXED_PER_MODEL * len(sets)
+    dec_ms = np.mean(np.array(decs), axis=0)
+    flops_m = [alg_flops(w["nw"] * w["np"], o[3]) for w, o in zip(sets, outs)]
+    flops = float(sum(flops_m))
+    achieved = flops / (float(dec_ms.sum()) * 1e-3) / 1e12
+    members = {f"ndim{w['nw'] * w['np']}": {
+        "solves_per_s_device": MIXED_PER_MODEL / (d * 1e-3), "decimate_ms": float(d),
+        "tflops_algorithmic": f / (d * 1e-3) / 1e12, "mean_iters": float(o[3].mean()),
+        "not_converged": int(np.count_nonzero(o[4] & 2))}
+        for w, d, f, o in zip(sets, dec_ms, flops_m, outs)}
+    log(f"[bench] mixed: {ninst / s:.1f} solves/s")
+    out = {
+        "metric": "G(k,w) solves/s, mixed models Ndim 8-216, per-instance eta (FP64)",
+        "value": ninst / s, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": s * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+        "config": {"workload": "surface GF per-instance batch: 4 synthetic models "
+                               "(Num_wann, Np) = (8,1), (16,2), (36,3), (72,3), R in [-2,2]^3, "
+                               f"{MIXED_PER_MODEL} instances each, eta log-uniform [1e-4, 1e-2] eV",
+                   "instances": ninst, "stop_rule": "sum|alpha|<=1e-16 Ha, itermax 100",
+                   "path": "fp64", "members": members},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "decimation stages of the four models: gj_round_kernel<1|2, double> + "
+                               "gemm_round_kernel<1|2, double> (Ndim 8, 32), gj_large_round_kernel<8|14, "
+                               "double> + gemm_large_round_kernel<8|14, double> (Ndim 108 padded to 128, "
+                               "216 padded to 224)",
+                     "flops_per_launch": flops},
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline ...")
+        per = args.cpu_per_worker or 6
+        models, jobs = [], []
+        for m, w in enumerate(sets):
+            models.append((w["nw"], w["np"], w["irvec"], w["hr"]))
+            pick = np.linspace(0, MIXED_PER_MODEL - 1, per * args.cpu_workers).astype(np.int64)
+            jobs += [(m, float(w["kpts"][w["k_index"][i], 0]), float(w["kpts"][w["k_index"][i], 1]),
+                      float(w["omega"][i]), float(w["eta"][i])) for i in pick]
+        out["cpu_baseline"] = cpu_baseline(models, jobs, args.cpu_workers,
+                                           "instances, equal numbers per model, strided over each "
+                                           "model's batch")
+    return out
+
+
+def main() -> int:
+    if len(sys.argv) == 7 and sys.argv[1] == "--summarize-rocprof":
+        return summarize_rocprof(*sys.argv[2:])
+    if len(sys.argv) >= 4 and sys.argv[1] == "--summarize-trace":
+        return summarize_trace(*sys.argv[2:])
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="grid72", choices=["grid72", "plane288", "mixed"])
+    ap.add_argument("--precision", default="auto", choices=["auto", "fp64", "fp32_refine"],
+                    help="grid72 only; the other workloads run FP64")
+    ap.add_argument("--nk", type=int, default=None,
+                    help="k points (grid72: on the path, default 512; plane288: per side, default 65)")
+    ap.add_argument("--nomega", type=int, default=None, help="grid72 energies (default 512)")
+    ap.add_argument("--cpu-workers", type=int, default=16)
+    ap.add_argument("--cpu-per-worker", type=int, default=None,
+                    help="CPU baseline solves per worker (default 400 grid72, 6 plane288, "
+                         "6 per model mixed)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
+    args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return 0
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the surface Green's function path is not "
+              "sharded across GPUs in this build (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    from wannier_tools_amd.surface import SurfaceGF
+
+    run = {"grid72": run_grid72, "plane288": run_plane288, "mixed": run_mixed}[args.workload]
+    out = run(args, SurfaceGF)
+    if "cpu_baseline" in out:
         log(f"[bench] cpu: {out['cpu_baseline']['value']:.1f} solves/s")
     print(json.dumps(out), flush=True)
     return 0

@@ -105,8 +105,9 @@ int wtsurf_spectral_batch(wtsurf_ctx *ctx, int nk, const double *kpts, int ninst
  * context streams in rounds and watches a device counter of finished
  * instances to know when to stop; it returns once the last rounds are
  * queued, shortly before the work completes.  wtsurf_grid_timing reports
- * the device time of the last run's Fourier and decimation stages (HIP
- * events on the context's first stream).  No reference counterpart. */
+ * the device time of the Fourier and decimation stages of the last grid run
+ * or wtsurf_spectral_batch call (HIP events on the context's first stream).
+ * No reference counterpart. */
 int wtsurf_grid_upload(wtsurf_ctx *ctx, int nk, const double *kpts, int nomega,
                        const double *omega);
 int wtsurf_grid_run_async(wtsurf_ctx *ctx, double eta, double divisor,

@@ -72,6 +72,9 @@ def test_batch_matches_grid(gpu):
     assert np.array_equal(br.reshape(shape), gr)
     assert np.array_equal(bb.reshape(shape), gb)
     assert np.array_equal(bit.reshape(shape), git)
+    # the batch's stage times are reported like a grid run's
+    ms_fourier, ms_decimate = gpu.timing()
+    assert ms_fourier > 0.0 and ms_decimate > 0.0
 
 
 def test_batch_rejects_bad_k_index(gpu):

@@ -84,7 +84,8 @@ struct wtsurf_ctx {
     DevBuf<unsigned char> work, state;
     DevBuf<unsigned> queue;
     int per_set = 0;
-    bool ran = false;
+    bool ran = false;    // a grid run is queued or done (wtsurf_grid_download)
+    bool timed = false;  // ev[0..2] bracket the last grid or batch run
 
     int fail(const std::string &m) {
         err = m;
@@ -457,6 +458,7 @@ int wtsurf_grid_run_async(wtsurf_ctx *c, double eta, double divisor, int precisi
         return r;
     WT_HIP(c, hipEventRecord(c->ev[2], c->stream));
     c->ran = true;
+    c->timed = true;
     return 0;
 }
 
@@ -470,7 +472,7 @@ int wtsurf_sync(wtsurf_ctx *c) {
 
 int wtsurf_grid_timing(wtsurf_ctx *c, float *ms_fourier, float *ms_decimate) {
     if (!c) return -1;
-    if (!c->ran) return c->fail("no completed run");
+    if (!c->timed) return c->fail("no completed run");
     float a = 0.f, b = 0.f;
     WT_HIP(c, hipEventSynchronize(c->ev[2]));
     WT_HIP(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
@@ -519,7 +521,10 @@ int wtsurf_spectral_batch(wtsurf_ctx *c, int nk, const double *kpts, int ninst, 
     c->nk = nk;
     c->nomega = 0;  // the grid no longer matches; wtsurf_grid_upload again for a grid run
     c->ran = false;
+    c->timed = false;
+    WT_HIP(c, hipEventRecord(c->ev[0], c->stream));
     if (int r = run_fourier(c)) return r;
+    WT_HIP(c, hipEventRecord(c->ev[1], c->stream));
     InstanceLists inst;
     inst.k = c->inst_k.p;
     inst.w = c->inst_w.p;
@@ -528,6 +533,8 @@ int wtsurf_spectral_batch(wtsurf_ctx *c, int nk, const double *kpts, int ninst, 
                                   c->d_top.p, (int)c->top.size(), c->d_bot.p, (int)c->bot.size(),
                                   precision, inst))
         return r;
+    WT_HIP(c, hipEventRecord(c->ev[2], c->stream));
+    c->timed = true;
     WT_HIP(c, hipStreamSynchronize(c->stream));
     const size_t ni = (size_t)ninst;
     WT_HIP(c, hipMemcpy(dos_l, c->dos_l.p, ni * sizeof(double), hipMemcpyDeviceToHost));

@@ -319,6 +319,8 @@ class SurfaceGF:
         self._check(self.lib.wtsurf_sync(self.ctx), "wtsurf_sync")
 
     def timing(self) -> tuple[float, float]:
+        """Device ms of the Fourier and decimation stages of the last grid run
+        or spectral_batch call."""
         a = ctypes.c_float()
         b = ctypes.c_float()
         self._check(self.lib.wtsurf_grid_timing(self.ctx, ctypes.byref(a), ctypes.byref(b)),
