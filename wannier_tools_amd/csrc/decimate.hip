@@ -75,8 +75,9 @@ struct GemmLay {
 template <int Q>
 struct Common {
     double2 diag[16 * Q];
-    int pivrow[16 * Q];   // pivot row of elimination step k
-    int pivstep[16 * Q];  // elimination step at which row i was the pivot (-1: not yet)
+    double2 rpiv[16 * Q];  // 1 / pivot of elimination step k (panel scratch)
+    int pivrow[16 * Q];    // pivot row of elimination step k
+    int pivstep[16 * Q];   // elimination step at which row i was the pivot (-1: not yet)
     int sing;
     SlotState s;          // the slot's state while the kernel runs
 };
@@ -93,6 +94,12 @@ struct Common {
 // panel (no barrier per step; the pivot search is a DPP wave maximum plus a
 // ballot).  Records the pivots in cm.pivrow / cm.pivstep and writes the panel
 // back.  Returns true (every lane) on an exactly zero pivot column.
+//
+// A row that becomes the pivot at step j is kept unscaled for the rest of the
+// panel: it holds u with u[j] = 1 and stands for r_j u (r_j = 1 / pivot).
+// Every later step acts on it linearly (its multiplier u[j'] r' is r_j^-1
+// times the scaled row's), so the panel runs the same update on every row
+// and scales each of its pivot rows by r_j once, at the write-back.
 template <int Q, typename R>
 __device__ __forceinline__ bool gj_panel(typename Prec<R>::C (*S)[16 * Q + 1], Common<Q> &cm,
                                          int c0, int n, int lane) {
@@ -157,29 +164,23 @@ __device__ __forceinline__ bool gj_panel(typename Prec<R>::C (*S)[16 * Q + 1], C
                 }
             }
         const C r = crecip_fast(row[j]);
-        // eliminate column c from every row with the multiplier m = f r
-        // (f = a[j], r = 1/pivot):  a[jj] = s a[jj] - m row[jj] for jj != j,
-        // a[j] = -m, s = 1.  The pivot row takes m = -r, s = 0, which leaves
-        // r row[jj] (the normalised pivot row) in it and r in entry j.
+        // eliminate column c from every other row with the multiplier
+        // m = f r (f = a[j]):  a[jj] -= m row[jj] for jj != j, a[j] = -m.
+        // The pivot row takes m = 0 and a[j] = 1.
 #pragma unroll
         for (int s = 0; s < NSET; ++s) {
             if (!live[s]) continue;
             const bool isp = slot == s && lane == pl;
             C m = cmul(a[s][j], r);
-            R sc = 1;
-            if (isp) {
-                m.x = -r.x;
-                m.y = -r.y;
-                sc = 0;
-            }
+            if (isp) m.x = m.y = 0;
 #pragma unroll
             for (int jj = 0; jj < kPanel; ++jj) {
                 if (jj == j) {
-                    a[s][jj].x = -m.x;
+                    a[s][jj].x = isp ? (R)1 : -m.x;
                     a[s][jj].y = -m.y;
                 } else {
-                    a[s][jj].x = sc * a[s][jj].x - (m.x * row[jj].x - m.y * row[jj].y);
-                    a[s][jj].y = sc * a[s][jj].y - (m.x * row[jj].y + m.y * row[jj].x);
+                    a[s][jj].x = fma(-m.x, row[jj].x, fma(m.y, row[jj].y, a[s][jj].x));
+                    a[s][jj].y = fma(-m.x, row[jj].y, fma(-m.y, row[jj].x, a[s][jj].y));
                 }
             }
             fr[s] = fr[s] && !isp;
@@ -187,13 +188,25 @@ __device__ __forceinline__ bool gj_panel(typename Prec<R>::C (*S)[16 * Q + 1], C
         if (lane == 0) {
             cm.pivrow[c] = p;
             cm.pivstep[p] = c;
+            cm.rpiv[c] = make_double2(r.x, r.y);
         }
     }
 #pragma unroll
     for (int s = 0; s < NSET; ++s)
-        if (has[s])
+        if (has[s]) {
+            const int r = lane + 64 * s;
+            const int ps = cm.pivstep[r];
+            if (ps >= c0 && ps < c0 + kPanel) {  // pivot row of this panel: scale by r_j
+                const double2 rp = cm.rpiv[ps];
+                C rr;
+                rr.x = (R)rp.x;
+                rr.y = (R)rp.y;
 #pragma unroll
-            for (int j = 0; j < kPanel; ++j) S[c0 + j][lane + 64 * s] = a[s][j];
+                for (int j = 0; j < kPanel; ++j) a[s][j] = cmul(a[s][j], rr);
+            }
+#pragma unroll
+            for (int j = 0; j < kPanel; ++j) S[c0 + j][r] = a[s][j];
+        }
     return sing;
 }
 
@@ -917,6 +930,75 @@ __device__ __forceinline__ void instance_of(const DecimateArgs &a, int t, int &i
 // GB = (z - eps_i)^-1 in FP64 and their traces, and the last of them writes
 // the instance's outputs.  sR / tR hold the matrix and the panel pivot rows
 // for the precision-R inverse, s64 / t64 for the FP64 final ones.
+// New-instance set-up of a slot in kEmpty, by NT threads: claim the next
+// instance from the queue (kDone past the last one) and start eps_s = eps_t =
+// H00, alpha = H01, beta = H01^dagger (src/surfgreen.f90:74-78), iteration 1.
+// st is the slot's state in shared memory; returns its new phase.
+template <int Q, typename R, int NT, typename W>
+__device__ __forceinline__ int setup_instance(const DecimateArgs &a, const W &w, SlotState &st) {
+    using C = typename Prec<R>::C;
+    constexpr int NP = 16 * Q;
+    constexpr int NN = NP * NP;
+    const int tid = threadIdx.x;
+    if (tid == 0) st.t = (int)atomicAdd(a.queue, 1u);
+    __syncthreads();
+    const int t = uniform(st.t);
+    if (t >= a.ninst) {
+        if (tid == 0) st.phase = kDone;
+        __syncthreads();
+        return kDone;
+    }
+    int ik;
+    double2 z;
+    instance_of(a, t, ik, z);
+    const double2 *H00 = a.hk + (size_t)ik * 2 * NN;
+    const double2 *H01 = H00 + NN;
+    C *al = w.alpha(0), *be = w.beta(0);
+    // batches whose loads are all issued before their stores (the stores
+    // may alias the loads as far as the compiler knows)
+    constexpr int kIter = (NN + NT - 1) / NT, kBatch = 4;
+    for (int it0 = 0; it0 < kIter; it0 += kBatch) {
+        double2 h[kBatch], h1[kBatch], ht[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) h[u] = h1[u] = ht[u] = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = tid + (it0 + u) * NT;
+            if (it0 + u < kIter && e < NN) {
+                const int i = e % NP, j = e / NP;
+                h[u] = H00[e];
+                h1[u] = H01[e];
+                ht[u] = H01[j + i * NP];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            const int e = tid + (it0 + u) * NT;
+            if (it0 + u < kIter && e < NN) {
+                w.es[e] = h[u];
+                w.et[e] = h[u];
+                C av;
+                av.x = (R)h1[u].x;
+                av.y = (R)h1[u].y;
+                al[e] = av;
+                C bv;
+                bv.x = (R)ht[u].x;
+                bv.y = (R)(-ht[u].y);
+                be[e] = bv;
+            }
+        }
+    }
+    if (tid == 0) {
+        st.it = 1;
+        st.status = 0;
+        st.dl = 0.0;
+        st.dr = 0.0;
+        st.phase = kLoop;
+    }
+    __syncthreads();
+    return kLoop;
+}
+
 template <int Q, typename R, int NT, typename W>
 __device__ __forceinline__ void gj_round(const DecimateArgs &a, int slot, const W &w,
                                          Common<Q> &cm, unsigned char *sR, unsigned char *tR,
@@ -933,61 +1015,13 @@ __device__ __forceinline__ void gj_round(const DecimateArgs &a, int slot, const 
     int phase = uniform(cm.s.phase);
     if (phase == kGemm || phase == kDone) return;
     if (phase == kEmpty) {
-        if (tid == 0) cm.s.t = (int)atomicAdd(a.queue, 1u);
-        __syncthreads();
-        const int t = uniform(cm.s.t);
-        if (t >= a.ninst) {
-            if (tid == 0) st->phase = kDone;
+        // only on a slot's first round: afterwards the GEMM round that follows
+        // an instance's last inverse starts the next one
+        phase = setup_instance<Q, R, NT>(a, w, cm.s);
+        if (phase == kDone) {
+            if (tid == 0) *st = cm.s;
             return;
         }
-        int ik;
-        double2 z;
-        instance_of(a, t, ik, z);
-        const double2 *H00 = a.hk + (size_t)ik * 2 * NN;
-        const double2 *H01 = H00 + NN;
-        C *al = w.alpha(0), *be = w.beta(0);
-        // batches whose loads are all issued before their stores (the stores
-        // may alias the loads as far as the compiler knows)
-        constexpr int kIter = (NN + NT - 1) / NT, kBatch = 4;
-        for (int it0 = 0; it0 < kIter; it0 += kBatch) {
-            double2 h[kBatch], h1[kBatch], ht[kBatch];
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u) h[u] = h1[u] = ht[u] = make_double2(0.0, 0.0);
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int e = tid + (it0 + u) * NT;
-                if (it0 + u < kIter && e < NN) {
-                    const int i = e % NP, j = e / NP;
-                    h[u] = H00[e];
-                    h1[u] = H01[e];
-                    ht[u] = H01[j + i * NP];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int e = tid + (it0 + u) * NT;
-                if (it0 + u < kIter && e < NN) {
-                    w.es[e] = h[u];
-                    w.et[e] = h[u];
-                    C av;
-                    av.x = (R)h1[u].x;
-                    av.y = (R)h1[u].y;
-                    al[e] = av;
-                    C bv;
-                    bv.x = (R)ht[u].x;
-                    bv.y = (R)(-ht[u].y);
-                    be[e] = bv;
-                }
-            }
-        }
-        if (tid == 0) {
-            cm.s.it = 1;
-            cm.s.status = 0;
-            cm.s.dl = 0.0;
-            cm.s.dr = 0.0;
-        }
-        phase = kLoop;
-        __syncthreads();
     }
     int ik;
     double2 z;
@@ -1082,19 +1116,28 @@ __global__ void __launch_bounds__(kBlock) gj_large_round_kernel(DecimateArgs a, 
 // The three dual GEMMs of one iteration for every slot in kGemm, then the
 // stop rule: kFinal0 when sum |alpha'| <= accuracy or at itermax (status bit
 // 1, iterations = itermax), else the next iteration's kLoop.  beta' and
-// alpha' overwrite beta and alpha in place.
+// alpha' overwrite beta and alpha in place.  A slot in kEmpty (its instance
+// finished in the inverse round before) is set up for its next instance
+// here, where the round has no GEMMs for it, instead of in the inverse round,
+// whose workgroups set the pace of the rounds.
 template <int Q, typename R>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_num_vgpr(kGemmVgprHalf)))
 gemm_round_kernel(DecimateArgs a, int slot0) {
     __shared__ __align__(16) unsigned char lds[GemmLay<Q, R>::bytes];
     __shared__ double red[kWaves];
-    __shared__ int sphase;
+    __shared__ SlotState sst;
     const int slot = slot0 + blockIdx.x;
     SlotState *st = a.state + slot;
-    if (threadIdx.x == 0) sphase = st->phase;
+    if (threadIdx.x == 0) sst = *st;
     __syncthreads();
-    if (uniform(sphase) != kGemm) return;
+    const int phase = uniform(sst.phase);
     const SlotWork<Q, R> w(a.work, slot);
+    if (phase == kEmpty) {
+        setup_instance<Q, R, kBlock>(a, w, sst);
+        if (threadIdx.x == 0) *st = sst;
+        return;
+    }
+    if (phase != kGemm) return;
     const int n = a.n;
     dual_gemm<Q, R, 0>(lds, red, w.al, w.be, w.g, n, w.m1, w.m2, nullptr);
     __syncthreads();
@@ -1102,7 +1145,7 @@ gemm_round_kernel(DecimateArgs a, int slot0) {
     __syncthreads();
     const double s = dual_gemm<Q, R, 2>(lds, red, w.m2, w.m1, w.al, n, nullptr, w.al, w.et);
     if (threadIdx.x == 0) {
-        SlotState v = *st;
+        SlotState v = sst;
         if (s <= a.accuracy) {
             v.phase = kFinal0;
         } else if (v.it >= a.itermax) {
@@ -1127,9 +1170,15 @@ __global__ void __launch_bounds__(kBlock) gemm_large_round_kernel(DecimateArgs a
     SlotState *st = a.state + slot;
     if (threadIdx.x == 0) sst = *st;
     __syncthreads();
-    if (uniform(sst.phase) != kGemm) return;
-    const int par = uniform((sst.it - 1) & 1);
+    const int phase = uniform(sst.phase);
     const SlotWorkL<Q, R> w(a.work, slot);
+    if (phase == kEmpty) {
+        setup_instance<Q, R, kBlock>(a, w, sst);
+        if (threadIdx.x == 0) *st = sst;
+        return;
+    }
+    if (phase != kGemm) return;
+    const int par = uniform((sst.it - 1) & 1);
     const int n = a.n;
     dual_gemm_blocked<Q, R, 0>(lds, red, w.alpha(par), w.beta(par), w.g, n, w.m1, w.m2, nullptr);
     __syncthreads();

@@ -56,15 +56,19 @@ __device__ __forceinline__ double fmax_nocanon(double a, double b) {
     return r;
 }
 
-// Maximum over the 64 lanes of a wave (operands as fmax_nocanon), returned
-// to every lane: DPP row shifts inside each row of 16 lanes, then
-// row broadcasts 15 and 31 carry the partial maxima up to lane 63.
+// Maximum over the 64 lanes of a wave of values whose maximum is >= 0
+// (operands as fmax_nocanon), returned to every lane: DPP row shifts inside
+// each row of 16 lanes, then row broadcasts 15 and 31 carry the partial
+// maxima up to lane 63.  A shift's lanes without a source read 0 (bound_ctrl),
+// which leaves a non-negative maximum as it is; the rows a broadcast does not
+// write hold unspecified values, but none of them reaches lane 63.  No old
+// value is kept, so no register copy precedes a DPP move.
 __device__ __forceinline__ double wave_max_f64(double v) {
 #define WT_DPP_MAX_STEP(ctrl, rmask)                                                  \
     {                                                                                  \
         const int hi = __double2hiint(v), lo = __double2loint(v);                      \
-        const int h2 = __builtin_amdgcn_update_dpp(hi, hi, ctrl, rmask, 0xf, false);   \
-        const int l2 = __builtin_amdgcn_update_dpp(lo, lo, ctrl, rmask, 0xf, false);   \
+        const int h2 = __builtin_amdgcn_mov_dpp(hi, ctrl, rmask, 0xf, true);           \
+        const int l2 = __builtin_amdgcn_mov_dpp(lo, ctrl, rmask, 0xf, true);           \
         v = fmax_nocanon(v, __hiloint2double(h2, l2));                                 \
     }
     WT_DPP_MAX_STEP(0x111, 0xf)  // row_shr:1
