@@ -428,8 +428,8 @@ def run_mixed(args, SurfaceGF) -> dict:
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
                      "kernel": "decimation stages of the four models: gj_round_kernel<1|2, double> + "
-                               "gemm_round_kernel<1|2, double> (Ndim 8, 32), gj_large_round_kernel<8|14, "
-                               "double> + gemm_large_round_kernel<8|14, double> (Ndim 108 padded to 128, "
+                               "gemm_round_kernel<1|2, double> (Ndim 8, 32), gj_large_round_kernel<7|14, "
+                               "double> + gemm_large_round_kernel<7|14, double> (Ndim 108 padded to 112, "
                                "216 padded to 224)",
                      "flops_per_launch": flops},
     }
