@@ -278,9 +278,16 @@ def run_grid72(args, SurfaceGF) -> dict:
                                                   ((dl, dl64), (dr, dr64), (db, db64))])))
             rel = np.concatenate([np.abs(x - y).ravel() / np.abs(y).ravel() for x, y in
                                   ((dl, dl64), (dr, dr64), (db, db64))])
+            per_file = {}
+            for name_, x, y in (("dos.dat_l", dl, dl64), ("dos.dat_r", dr, dr64),
+                                ("dos.dat_bulk", db, db64)):
+                r_ = (np.abs(x - y) / np.abs(y)).ravel()
+                per_file[name_] = {"max_rel_err_A": float(r_.max()),
+                                   "median_rel_err_A": float(np.median(r_))}
             res["fp32_refine"] = {"s_per_step": s32, "solves_per_s": ninst / s32, "decimate_ms": dec32,
                                   "max_rel_err_A": err, "median_rel_err_A": med,
                                   "frac_within_1e-6": float(np.mean(rel <= 1e-6)),
+                                  "per_file": per_file,
                                   "speedup_vs_fp64": s64 / s32}
             log(f"[bench] fp32_refine: {ninst / s32:.1f} solves/s, max rel err {err:.2e}")
             if err <= 1e-6 and s32 < best[1]:
