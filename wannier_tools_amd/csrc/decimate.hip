@@ -264,6 +264,67 @@ __device__ __forceinline__ void gj_update_tile(typename Prec<R>::C (*S)[16 * Q +
     }
 }
 
+// Rank-2kPanel MFMA update of tile (ti, tj) of S for the panel pair p (columns
+// c0 .. c0+kPanel-1) and q (the next kPanel), applied at once (orders beyond
+// 80, whose S lives in global memory: one read-modify-write of the matrix
+// per pair instead of per panel).  T holds p's pivot rows in rows 0..kPanel-1
+// and q's, taken after p's update, in rows kPanel..2kPanel-1; Mp holds p's
+// columns of S as they were before q's update.  Per element, in the order the
+// two single-panel updates would give it:
+//   rows pivoted in q:  M_q T_q (their old content is in T_q, p's term drops),
+//   rows pivoted in p:  M_p T_p + M_q T_q, except that in p's own columns they
+//                       keep their content (the p term is zero there),
+//   other rows:         S + M_p T_p + M_q T_q;
+// q's columns are left as q's elimination left them.
+template <int Q, typename R>
+__device__ __forceinline__ void gj_update_tile2(typename Prec<R>::C (*S)[16 * Q + 1],
+                                                typename Prec<R>::C (*T)[16 * Q],
+                                                typename Prec<R>::C (*Mp)[16 * Q],
+                                                const Common<Q> &cm, int ti, int tj, int c0,
+                                                int lr, int lq) {
+    using P = Prec<R>;
+    using C = typename P::C;
+    const int c1 = c0 + kPanel;
+    typename P::V4 accR = {0, 0, 0, 0}, accI = {0, 0, 0, 0};
+    const int arow = 16 * ti + lr;
+    const int aps = cm.pivstep[arow];
+    const bool drop_p = aps >= c1 && aps < c1 + kPanel;  // pivoted in q
+#pragma unroll
+    for (int ks = 0; ks < kPanel / 4; ++ks) {
+        C a = Mp[4 * ks + lq][arow];
+        if (drop_p) a.x = a.y = 0;
+        const C b = T[4 * ks + lq][16 * tj + lr];
+        accR = P::mfma(a.x, b.x, accR);
+        accR = P::mfma(-a.y, b.y, accR);
+        accI = P::mfma(a.x, b.y, accI);
+        accI = P::mfma(a.y, b.x, accI);
+    }
+#pragma unroll
+    for (int ks = 0; ks < kPanel / 4; ++ks) {
+        const C a = S[c1 + 4 * ks + lq][arow];
+        const C b = T[kPanel + 4 * ks + lq][16 * tj + lr];
+        accR = P::mfma(a.x, b.x, accR);
+        accR = P::mfma(-a.y, b.y, accR);
+        accI = P::mfma(a.x, b.y, accI);
+        accI = P::mfma(a.y, b.x, accI);
+    }
+    const int col = 16 * tj + lr;
+    if (col < c1 || col >= c1 + kPanel) {
+        const bool pcol = col >= c0 && col < c1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int row = 16 * ti + P::out_row(lq, v);
+            const int ps = cm.pivstep[row];
+            const bool zero = (ps >= c1 && ps < c1 + kPanel) || (!pcol && ps >= c0 && ps < c1);
+            C o = S[col][row];
+            if (zero) o.x = o.y = 0;
+            o.x += accR[v];
+            o.y += accI[v];
+            S[col][row] = o;
+        }
+    }
+}
+
 // Inverse of X = z*1 - E (the first n x n block of the padded FP64 matrix E,
 // or of E + E2 - H when E2 is given) in precision R, by NT threads.  X is
 // held in sbuf (NP x (NP+1), LDS for orders up to 80, the slot's global
@@ -352,6 +413,80 @@ __device__ __forceinline__ bool gj_inverse(unsigned char *sbuf, unsigned char *t
         if (lane == 0 && sing) cm.sing = 1;
     }
     __syncthreads();
+    if constexpr (Q > 5) {
+        // S in global memory: panels in pairs, one rank-2kPanel update each
+        C(*Mp)[NP] = reinterpret_cast<C(*)[NP]>(tbuf + 2 * kPanel * NP * sizeof(C));
+        for (int pb = 0; pb < npanel; pb += 2) {
+            if (uniform(cm.sing)) break;
+            const int c0 = pb * kPanel, c1 = c0 + kPanel;
+            gj_gather_t<Q, R, NT>(S, T, cm, c0, n, tid);
+            if (pb + 1 >= npanel) {  // a last panel of its own
+                __syncthreads();
+#pragma unroll 1
+                for (int t = wave; t < Q * Q; t += NWAVES)
+                    gj_update_tile<Q, R>(S, T, cm, t % Q, t / Q, c0, lr, lq);
+                __syncthreads();
+                break;
+            }
+            for (int e = tid; e < kPanel * NP; e += NT) Mp[e / NP][e % NP] = S[c0 + e / NP][e % NP];
+            __syncthreads();
+            // q's columns (tile column c0 / 16, outside p's) updated with p,
+            // then q eliminated
+            const int tp = c0 / 16;
+#pragma unroll 1
+            for (int ti = wave; ti < Q; ti += NWAVES) gj_update_tile<Q, R>(S, T, cm, ti, tp, c0, lr, lq);
+            __syncthreads();
+            if (wave == 0) {
+                const bool sing = gj_panel<Q, R>(S, cm, c1, n, lane);
+                if (lane == 0 && sing) cm.sing = 1;
+            }
+            __syncthreads();
+            if (uniform(cm.sing)) break;
+            // q's pivot rows after p's update, outside q's columns
+            for (int e = tid; e < kPanel * NP; e += NT) {
+                const int k = e / NP, col = e % NP;
+                C tv;
+                tv.x = 0;
+                tv.y = 0;
+                if (c1 + k < n && (col < c1 || col >= c1 + kPanel)) {
+                    const int r = cm.pivrow[c1 + k];
+                    tv = S[col][r];
+                    if (col < c0 || col >= c1)
+#pragma unroll
+                        for (int j = 0; j < kPanel; ++j) {
+                            const C m = Mp[j][r], t = T[j][col];
+                            tv.x += m.x * t.x - m.y * t.y;
+                            tv.y += m.x * t.y + m.y * t.x;
+                        }
+                }
+                T[kPanel + k][col] = tv;
+            }
+            __syncthreads();
+            // the pair's update: first the tile column of the next pair, which
+            // wave 0 then eliminates while the other waves update the rest
+            const bool ahead = pb + 2 < npanel;
+            const int tn = tp + 1;
+            if (ahead) {
+#pragma unroll 1
+                for (int ti = wave; ti < Q; ti += NWAVES)
+                    gj_update_tile2<Q, R>(S, T, Mp, cm, ti, tn, c0, lr, lq);
+                __syncthreads();
+            }
+            if (ahead && wave == 0) {
+                const bool sing = gj_panel<Q, R>(S, cm, c1 + kPanel, n, lane);
+                if (lane == 0 && sing) cm.sing = 1;
+            } else {
+                const int first = ahead ? 1 : 0, nw = ahead ? NWAVES - 1 : NWAVES;
+#pragma unroll 1
+                for (int t = wave - first; t < Q * Q; t += nw) {
+                    const int ti = t % Q, tj = t / Q;
+                    if (ahead && tj == tn) continue;
+                    gj_update_tile2<Q, R>(S, T, Mp, cm, ti, tj, c0, lr, lq);
+                }
+            }
+            __syncthreads();
+        }
+    } else
     for (int pb = 0; pb < npanel; ++pb) {
         if (uniform(cm.sing)) break;
         const int c0 = pb * kPanel;
@@ -1104,9 +1239,16 @@ __global__ void __launch_bounds__(kGjBlock) gj_round_kernel(DecimateArgs a, int 
 
 // Orders beyond 80: the matrix in the slot's global workspace (it no longer
 // fits in LDS), the panel pivot rows in LDS, 8 waves.
+// Its LDS (dynamic, GjLayL<Q>::bytes): the pivot rows of a panel pair and the
+// first panel's columns (3 kPanel rows of 16 Q, FP64 at most).
+template <int Q>
+struct GjLayL {
+    static constexpr size_t bytes = (size_t)3 * kPanel * 16 * Q * sizeof(double2);
+};
+
 template <int Q, typename R>
 __global__ void __launch_bounds__(kBlock) gj_large_round_kernel(DecimateArgs a, int slot0) {
-    __shared__ __align__(16) unsigned char tlds[(size_t)kPanel * 16 * Q * sizeof(double2)];
+    extern __shared__ __align__(16) unsigned char tlds[];
     __shared__ Common<Q> cm;
     const int slot = slot0 + blockIdx.x;
     const SlotWorkL<Q, R> w(a.work, slot);
@@ -1221,9 +1363,10 @@ static int launch_round(const DecimateArgs &a, int per_set, int set, bool gj, hi
             hipLaunchKernelGGL((gemm_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), 0, s, a,
                                set * per_set);
     } else {
+        constexpr size_t gj_lds = GjLayL<Q>::bytes;
         if (gj)
-            hipLaunchKernelGGL((gj_large_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), 0, s, a,
-                               set * per_set);
+            hipLaunchKernelGGL((gj_large_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), gj_lds, s,
+                               a, set * per_set);
         else
             hipLaunchKernelGGL((gemm_large_round_kernel<Q, R>), dim3(per_set), dim3(kBlock), 0, s, a,
                                set * per_set);
@@ -1261,20 +1404,23 @@ static size_t dispatch_slot_bytes(int q) {
 // reports whether the size is accepted.
 template <int Q, typename R>
 static void allow_lds() {
-    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&gj_round_kernel<Q, R>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)GjLay<Q, R>::bytes) != hipSuccess)
-        (void)hipGetLastError();
+    hipError_t e;
+    if constexpr (Q <= 5)
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&gj_round_kernel<Q, R>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)GjLay<Q, R>::bytes);
+    else
+        e = hipFuncSetAttribute(reinterpret_cast<const void *>(&gj_large_round_kernel<Q, R>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)GjLayL<Q>::bytes);
+    if (e != hipSuccess) (void)hipGetLastError();
 }
 
 template <typename R>
 static void dispatch_allow_lds(int q) {
     switch (q) {
-        case 1: allow_lds<1, R>(); break;
-        case 2: allow_lds<2, R>(); break;
-        case 3: allow_lds<3, R>(); break;
-        case 4: allow_lds<4, R>(); break;
-        case 5: allow_lds<5, R>(); break;
+#define WTSURF_CASE(Q) \
+    case Q: allow_lds<Q, R>(); break;
+        WTSURF_FOR_EACH_Q(WTSURF_CASE)
+#undef WTSURF_CASE
         default: break;
     }
 }
