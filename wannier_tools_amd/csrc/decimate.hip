@@ -276,12 +276,32 @@ __device__ __forceinline__ void gj_update_tile(typename Prec<R>::C (*S)[16 * Q +
 //                       keep their content (the p term is zero there),
 //   other rows:         S + M_p T_p + M_q T_q;
 // q's columns are left as q's elimination left them.
+// A tile's global-memory operands for gj_update_tile2: q's columns of S in the
+// tile's rows (the MFMA A operand) and the tile's old values.
+template <typename C>
+struct Tile2Ops {
+    C aq[kPanel / 4];
+    C old[4];
+};
+
 template <int Q, typename R>
-__device__ __forceinline__ void gj_update_tile2(typename Prec<R>::C (*S)[16 * Q + 1],
+__device__ __forceinline__ void gj_tile2_load(typename Prec<R>::C (*S)[16 * Q + 1], int ti, int tj,
+                                              int c1, int lr, int lq,
+                                              Tile2Ops<typename Prec<R>::C> &o) {
+#pragma unroll
+    for (int ks = 0; ks < kPanel / 4; ++ks) o.aq[ks] = S[c1 + 4 * ks + lq][16 * ti + lr];
+    const int col = 16 * tj + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o.old[v] = S[col][16 * ti + Prec<R>::out_row(lq, v)];
+}
+
+template <int Q, typename R>
+__device__ __forceinline__ void gj_tile2_finish(typename Prec<R>::C (*S)[16 * Q + 1],
                                                 typename Prec<R>::C (*T)[16 * Q],
                                                 typename Prec<R>::C (*Mp)[16 * Q],
                                                 const Common<Q> &cm, int ti, int tj, int c0,
-                                                int lr, int lq) {
+                                                int lr, int lq,
+                                                const Tile2Ops<typename Prec<R>::C> &o) {
     using P = Prec<R>;
     using C = typename P::C;
     const int c1 = c0 + kPanel;
@@ -301,7 +321,7 @@ __device__ __forceinline__ void gj_update_tile2(typename Prec<R>::C (*S)[16 * Q 
     }
 #pragma unroll
     for (int ks = 0; ks < kPanel / 4; ++ks) {
-        const C a = S[c1 + 4 * ks + lq][arow];
+        const C a = o.aq[ks];
         const C b = T[kPanel + 4 * ks + lq][16 * tj + lr];
         accR = P::mfma(a.x, b.x, accR);
         accR = P::mfma(-a.y, b.y, accR);
@@ -316,12 +336,52 @@ __device__ __forceinline__ void gj_update_tile2(typename Prec<R>::C (*S)[16 * Q 
             const int row = 16 * ti + P::out_row(lq, v);
             const int ps = cm.pivstep[row];
             const bool zero = (ps >= c1 && ps < c1 + kPanel) || (!pcol && ps >= c0 && ps < c1);
-            C o = S[col][row];
-            if (zero) o.x = o.y = 0;
-            o.x += accR[v];
-            o.y += accI[v];
-            S[col][row] = o;
+            C x = o.old[v];
+            if (zero) x.x = x.y = 0;
+            x.x += accR[v];
+            x.y += accI[v];
+            S[col][row] = x;
         }
+    }
+}
+
+// Rank-2kPanel MFMA update for the panel pair p (columns c0 .. c0+kPanel-1)
+// and q (the next kPanel), applied at once to the tiles t = first, first +
+// stride, ... < Q*Q (t = tj Q + ti) other than those in tile column skip_tj
+// (orders beyond 80, whose S lives in global memory: one read-modify-write of
+// the matrix per pair instead of per panel).  T holds p's pivot rows in rows
+// 0..kPanel-1 and q's, taken after p's update, in rows kPanel..2kPanel-1; Mp
+// holds p's columns of S as they were before q's update.  Per element, in the
+// order the two single-panel updates would give it:
+//   rows pivoted in q:  M_q T_q (their old content is in T_q, p's term drops),
+//   rows pivoted in p:  M_p T_p + M_q T_q, except that in p's own columns they
+//                       keep their content (the p term is zero there),
+//   other rows:         S + M_p T_p + M_q T_q;
+// q's columns are left as q's elimination left them.  Each tile's global
+// loads are issued before the previous tile is finished, so the memory
+// latency of a wave's tiles overlaps the MFMAs and stores of the tile before.
+template <int Q, typename R>
+__device__ __forceinline__ void gj_update_tiles2(typename Prec<R>::C (*S)[16 * Q + 1],
+                                                 typename Prec<R>::C (*T)[16 * Q],
+                                                 typename Prec<R>::C (*Mp)[16 * Q],
+                                                 const Common<Q> &cm, int first, int stride,
+                                                 int skip_tj, int c0, int lr, int lq) {
+    using C = typename Prec<R>::C;
+    auto next = [&](int t) {
+        while (t < Q * Q && t / Q == skip_tj) t += stride;
+        return t;
+    };
+    int t = next(first);
+    if (t >= Q * Q) return;
+    Tile2Ops<C> cur, nxt;
+    gj_tile2_load<Q, R>(S, t % Q, t / Q, c0 + kPanel, lr, lq, cur);
+    for (;;) {
+        const int tn = next(t + stride);
+        if (tn < Q * Q) gj_tile2_load<Q, R>(S, tn % Q, tn / Q, c0 + kPanel, lr, lq, nxt);
+        gj_tile2_finish<Q, R>(S, T, Mp, cm, t % Q, t / Q, c0, lr, lq, cur);
+        if (tn >= Q * Q) break;
+        cur = nxt;
+        t = tn;
     }
 }
 
@@ -467,9 +527,12 @@ __device__ __forceinline__ bool gj_inverse(unsigned char *sbuf, unsigned char *t
             const bool ahead = pb + 2 < npanel;
             const int tn = tp + 1;
             if (ahead) {
-#pragma unroll 1
-                for (int ti = wave; ti < Q; ti += NWAVES)
-                    gj_update_tile2<Q, R>(S, T, Mp, cm, ti, tn, c0, lr, lq);
+                // tiles ti = wave, wave + NWAVES, ... of tile column tn
+                for (int ti = wave; ti < Q; ti += NWAVES) {
+                    Tile2Ops<C> ops;
+                    gj_tile2_load<Q, R>(S, ti, tn, c1, lr, lq, ops);
+                    gj_tile2_finish<Q, R>(S, T, Mp, cm, ti, tn, c0, lr, lq, ops);
+                }
                 __syncthreads();
             }
             if (ahead && wave == 0) {
@@ -477,12 +540,7 @@ __device__ __forceinline__ bool gj_inverse(unsigned char *sbuf, unsigned char *t
                 if (lane == 0 && sing) cm.sing = 1;
             } else {
                 const int first = ahead ? 1 : 0, nw = ahead ? NWAVES - 1 : NWAVES;
-#pragma unroll 1
-                for (int t = wave - first; t < Q * Q; t += nw) {
-                    const int ti = t % Q, tj = t / Q;
-                    if (ahead && tj == tn) continue;
-                    gj_update_tile2<Q, R>(S, T, Mp, cm, ti, tj, c0, lr, lq);
-                }
+                gj_update_tiles2<Q, R>(S, T, Mp, cm, wave - first, nw, ahead ? tn : -1, c0, lr, lq);
             }
             __syncthreads();
         }
@@ -845,11 +903,17 @@ __device__ __forceinline__ double dual_gemm(unsigned char *lds, double *red,
 // LDS of the blocked GEMM (orders beyond 80): one K-chunk of the A0 and A1
 // row blocks (16 BI rows) and of the B column block (16 BJ columns, padded).
 constexpr int kBI = 6, kBJ = 4;  // output block of the blocked GEMM, in tiles
+// K-chunk of the blocked GEMM: twice the small-order one, since its workgroup
+// does not share the CU with an inverse workgroup (the two kernels' registers
+// do not fit one CU together at these orders) and so has the LDS to spare;
+// a deeper chunk halves the barriers per K and gives each chunk's loads more
+// MFMA work to hide behind.
+constexpr int kChunkL = 16;
 template <int Q, typename R>
 struct GemmLayL {
     using C = typename Prec<R>::C;
-    static constexpr size_t A = (size_t)kChunk * 16 * kBI * sizeof(C);
-    static constexpr size_t B = (size_t)kChunk * (16 * kBJ + 1) * sizeof(C);
+    static constexpr size_t A = (size_t)kChunkL * 16 * kBI * sizeof(C);
+    static constexpr size_t B = (size_t)kChunkL * (16 * kBJ + 1) * sizeof(C);
     static constexpr size_t bytes = 2 * A + B;
 };
 
@@ -874,21 +938,18 @@ __device__ __forceinline__ double dual_gemm_blocked(unsigned char *lds, double *
     constexpr int NBI = (Q + kBI - 1) / kBI, NBJ = (Q + kBJ - 1) / kBJ;
     constexpr int PPW = kBI * kBJ / kWaves;
     static_assert(PPW * kWaves == kBI * kBJ, "block pairs must split evenly over the waves");
-    static_assert(kChunk * CJ == kBlock, "one B element per thread and chunk");
-    static_assert(kChunk * RI <= 2 * kBlock, "at most two A elements per thread and chunk");
+    constexpr int NA = kChunkL * RI / kBlock, NB = kChunkL * CJ / kBlock;
+    static_assert(NA * kBlock == kChunkL * RI, "whole A elements per thread and chunk");
+    static_assert(NB * kBlock == kChunkL * CJ, "whole B elements per thread and chunk");
     const int tid = threadIdx.x;
     const int wave = uniform(tid >> 6), lane = tid & 63;
     const int lr = lane & 15, lq = lane >> 4;
     C(*as0)[RI] = reinterpret_cast<C(*)[RI]>(lds);
     C(*as1)[RI] = reinterpret_cast<C(*)[RI]>(lds + GemmLayL<Q, R>::A);
     C(*bs)[CJ + 1] = reinterpret_cast<C(*)[CJ + 1]>(lds + 2 * GemmLayL<Q, R>::A);
-    const int nchunks = (kdim + kChunk - 1) / kChunk;
-    // staging: A elements e = tid, tid + kBlock of a chunk (kk = e / RI, row e % RI),
-    // B element tid (kk = tid % kChunk, column tid / kChunk)
-    const int ea1 = tid + kBlock;
-    const bool ha1 = ea1 < kChunk * RI;
-    const int ka0 = tid / RI, ra0 = tid % RI, ka1 = ha1 ? ea1 / RI : 0, ra1 = ea1 % RI;
-    const int kb = tid % kChunk, cb = tid / kChunk;
+    const int nchunks = (kdim + kChunkL - 1) / kChunkL;
+    // staging: A elements e = tid + u kBlock of a chunk (kk = e / RI, row e % RI),
+    // B elements e = tid + u kBlock (kk = e % kChunkL, column e / kChunkL)
 
     double asum = 0.0;
     for (int blk = 0; blk < NBI * NBJ; ++blk) {
@@ -909,30 +970,37 @@ __device__ __forceinline__ double dual_gemm_blocked(unsigned char *lds, double *
                 pI[s][w] = V4{0, 0, 0, 0};
             }
         }
-        const bool va0 = 16 * i0 + ra0 < NP, va1 = ha1 && 16 * i0 + ra1 < NP;
-        const bool vb = 16 * j0 + cb < NP;
-        C x0, y0, x1, y1, b0;
+        C xa[NA], ya[NA], bb[NB];
         auto load = [&](int c) {
-            const int k0 = c * kChunk;
-            x0.x = x0.y = y0.x = y0.y = x1.x = x1.y = y1.x = y1.y = b0.x = b0.y = 0;
-            if (va0) {
-                x0 = A0[16 * i0 + ra0 + (k0 + ka0) * NP];
-                y0 = A1[16 * i0 + ra0 + (k0 + ka0) * NP];
+            const int k0 = c * kChunkL;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                const int e = tid + u * kBlock, ka = e / RI, ra = e % RI;
+                xa[u].x = xa[u].y = ya[u].x = ya[u].y = 0;
+                if (16 * i0 + ra < NP) {
+                    xa[u] = A0[16 * i0 + ra + (k0 + ka) * NP];
+                    ya[u] = A1[16 * i0 + ra + (k0 + ka) * NP];
+                }
             }
-            if (va1) {
-                x1 = A0[16 * i0 + ra1 + (k0 + ka1) * NP];
-                y1 = A1[16 * i0 + ra1 + (k0 + ka1) * NP];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const int e = tid + u * kBlock, kb = e % kChunkL, cb = e / kChunkL;
+                bb[u].x = bb[u].y = 0;
+                if (16 * j0 + cb < NP) bb[u] = B[k0 + kb + (16 * j0 + cb) * NP];
             }
-            if (vb) b0 = B[k0 + kb + (16 * j0 + cb) * NP];
         };
         auto stash = [&]() {
-            as0[ka0][ra0] = x0;
-            as1[ka0][ra0] = y0;
-            if (ha1) {
-                as0[ka1][ra1] = x1;
-                as1[ka1][ra1] = y1;
+#pragma unroll
+            for (int u = 0; u < NA; ++u) {
+                const int e = tid + u * kBlock;
+                as0[e / RI][e % RI] = xa[u];
+                as1[e / RI][e % RI] = ya[u];
             }
-            bs[kb][cb] = b0;
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const int e = tid + u * kBlock;
+                bs[e % kChunkL][e / kChunkL] = bb[u];
+            }
         };
         load(0);
         for (int c = 0; c < nchunks; ++c) {
@@ -940,7 +1008,7 @@ __device__ __forceinline__ double dual_gemm_blocked(unsigned char *lds, double *
             __syncthreads();
             if (c + 1 < nchunks) load(c + 1);
 #pragma unroll
-            for (int ks = 0; ks < kChunk / 4; ++ks) {
+            for (int ks = 0; ks < kChunkL / 4; ++ks) {
                 const int kk = 4 * ks + lq;
 #pragma unroll
                 for (int s = 0; s < PPW; ++s) {
