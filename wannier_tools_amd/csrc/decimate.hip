@@ -902,7 +902,7 @@ __device__ __forceinline__ double dual_gemm(unsigned char *lds, double *red,
 
 // LDS of the blocked GEMM (orders beyond 80): one K-chunk of the A0 and A1
 // row blocks (16 BI rows) and of the B column block (16 BJ columns, padded).
-constexpr int kBI = 6, kBJ = 4;  // output block of the blocked GEMM, in tiles
+constexpr int kBI = 4, kBJ = 6;  // output block of the blocked GEMM, in tiles
 // K-chunk of the blocked GEMM: twice the small-order one, since its workgroup
 // does not share the CU with an inverse workgroup (the two kernels' registers
 // do not fit one CU together at these orders) and so has the LDS to spare;
