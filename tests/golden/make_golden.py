@@ -6,7 +6,7 @@ its ham_qlayer2qlayer and surfgreen_1985 through the C-ABI harness
 (oracle/ref_harness.f90).  The outputs are stored as .npz fixtures so the
 tests run without the reference (and on machines that never see it).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--only ndim216,ndim288]
 """
 from __future__ import annotations
 
@@ -53,7 +53,8 @@ def traces(gll, grr, gb, nw, top, bot, divisor=1.0):
 
 
 def run_case(ref: oracle.RefHarness, name: str, mdl: M.TBModel, np_: int, kpts, omegas_ev,
-             eta_ev: float, store_model: bool, full_first: bool = False) -> None:
+             eta_ev: float, store_model: bool, full_first: bool = False,
+             store_h: bool = True) -> None:
     irvec, hr = mdl.surface_cell()
     nw = mdl.num_wann
     n = nw * np_
@@ -81,9 +82,11 @@ def run_case(ref: oracle.RefHarness, name: str, mdl: M.TBModel, np_: int, kpts, 
             tr[:, ik, iw] = traces(gll, grr, gb, nw, top, top)
             if full_first and full is None:
                 full = np.stack([gll, grr, gb])
-    payload = dict(nw=nw, np=np_, kpts=kpts, omegas=omegas, eta=eta, h00=h00, h01=h01,
+    payload = dict(nw=nw, np=np_, kpts=kpts, omegas=omegas, eta=eta,
                    diag_gll=dgll, diag_grr=dgrr, diag_gb=dgb, dos_l=tr[0], dos_r=tr[1],
                    dos_bulk=tr[2], model_sha256=model_digest(irvec, hr))
+    if store_h:
+        payload.update(h00=h00, h01=h01)
     if store_model:
         payload.update(irvec=irvec, hr=hr)
     if full is not None:
@@ -93,9 +96,18 @@ def run_case(ref: oracle.RefHarness, name: str, mdl: M.TBModel, np_: int, kpts, 
     print(f"wrote {path} ({os.path.getsize(path)} bytes)")
 
 
-def main() -> None:
+def mixed_ndim216_model() -> M.TBModel:
+    """The Ndim = 216 member of the mixed-model set: 72 orbitals, R in [-2,2]^3
+    (Np = 3)."""
+    return M.synthetic_model(72, 2, 0.5, M.decay_l2(1.5), seed=43)
+
+
+def main(only=None) -> None:
     oracle.build(ref=True)
     ref = oracle.RefHarness()
+    if only:
+        large(ref, only)
+        return
     # Ndim = 16 (8 orbitals, Np = 2): k on the (-0.5,0)->(0.5,0) path, omega on
     # the surfstat grid of [-1, 1] eV with 401 points, eta = 1e-3 eV.
     kp = M.kpath_slab([((-0.5, 0.0), (0.5, 0.0))], 201)[[0, 37, 100, 163, 200]]
@@ -109,7 +121,22 @@ def main() -> None:
     kp3 = np.array([[0.11, -0.23], [0.37, 0.41]])
     run_case(ref, "np3_ndim12", small_np3_model(), 3, kp3, [-0.4, 0.05, 0.6], 5e-3,
              store_model=True)
+    large(ref, ("ndim216", "ndim288"))
+
+
+def large(ref, names) -> None:
+    """Orders beyond 80 (the kernels with the matrix in global memory).  No
+    H00 / H01 (megabytes at these orders): the tests form them on the device,
+    whose Fourier sum the smaller cases pin."""
+    if "ndim216" in names:
+        # the mixed-model set's largest member, k off the symmetry lines
+        run_case(ref, "ndim216", mixed_ndim216_model(), 3, [[0.13, -0.21]], [-0.3, 0.05], 2e-3,
+                 store_model=False, store_h=False)
+    if "ndim288" in names:
+        # the 144-orbital spinful k-plane model at omega = 0, eta = 5e-4 eV
+        run_case(ref, "ndim288", M.model_ndim288(), 2, [[-0.25, 0.125], [0.3125, -0.4375]], [0.0],
+                 5e-4, store_model=False, store_h=False)
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[2].split(",") if len(sys.argv) > 2 and sys.argv[1] == "--only" else None)

@@ -13,13 +13,29 @@ def model_digest(irvec, hr) -> str:
     return h.hexdigest()
 
 
+def mixed_ndim216_model():
+    """The Ndim = 216 member of the mixed-model set (72 orbitals, R in
+    [-2,2]^3, Np = 3), as tests/golden/make_golden.py builds it."""
+    return M.synthetic_model(72, 2, 0.5, M.decay_l2(1.5), seed=43)
+
+
+# Golden cases beyond Ndim 80 (no H00 / H01 stored) and their tolerances: about
+# five times the oracle's own distance from the reference on them, which is
+# 1-2 times the cases' FP64 floor (traces: 3.4e-9 / floor 2.0e-9 at Ndim 216,
+# 3.9e-11 / floor 7.7e-11 at Ndim 288; single diagonal entries of G: 1.9e-8
+# and 6.2e-10).
+LARGE_GOLDEN = {"ndim216": dict(trace=1e-8, diag=1e-7),
+                "ndim288": dict(trace=5e-10, diag=3e-9)}
+
+
 def golden_model(g, name):
     """(nw, np, irvec, hr) of a golden case; regenerated models are checked
     against the digest stored with the fixture."""
     if "irvec" in g:
         irvec, hr = g["irvec"], g["hr"]
     else:
-        mdl = {"ndim72": M.model_ndim72}[name]()
+        mdl = {"ndim72": M.model_ndim72, "ndim216": mixed_ndim216_model,
+               "ndim288": M.model_ndim288}[name]()
         irvec, hr = mdl.surface_cell()
     assert model_digest(irvec, hr) == str(g["model_sha256"]), "synthetic model generator changed"
     return int(g["nw"]), int(g["np"]), irvec, hr

@@ -12,7 +12,7 @@ there than that.
 import numpy as np
 import pytest
 
-from helpers import chain_bulk_g, chain_surface_g, fp64_floor, golden_model, max_rel
+from helpers import LARGE_GOLDEN, chain_bulk_g, chain_surface_g, fp64_floor, golden_model, max_rel
 from wannier_tools_amd import model as M
 from wannier_tools_amd.surface import STATUS_NOT_CONVERGED, STATUS_SINGULAR, WtsurfError
 
@@ -81,6 +81,30 @@ def test_spectral_grid_vs_reference_golden(gpu, golden, oracle_lib, name):
     top = M.default_top_orbitals(nw)
     *_, oits = oracle_lib.spectral(nw, np_, irvec, hr, g["kpts"], g["omegas"], float(g["eta"]), top, top)
     assert np.array_equal(its, oits)
+
+
+@pytest.mark.parametrize("name", sorted(LARGE_GOLDEN))
+def test_large_orders_vs_reference_golden(gpu, golden, name):
+    """Orders beyond 80 (matrix in global memory, panel pairs, blocked GEMMs)
+    against the reference's own surfgreen_1985: the spectral traces, and the
+    diagonals of GLL, GRR, GB from surfgreen_1985 on the device's H00 / H01."""
+    _say("large golden", name)
+    g = golden(name)
+    nw, np_, irvec, hr = golden_model(g, name)
+    tol = LARGE_GOLDEN[name]
+    gpu.set_model_arrays(nw, np_, irvec, hr)
+    dl, dr, db, its, st = gpu.spectral_grid(g["kpts"], g["omegas"], float(g["eta"]))
+    errs = [max_rel(dl, g["dos_l"]), max_rel(dr, g["dos_r"]), max_rel(db, g["dos_bulk"])]
+    _say("  max rel err dos_l, dos_r, dos_bulk vs reference:", errs)
+    assert max(errs) < tol["trace"]
+    assert np.all(st == 0)
+    h00, h01 = gpu.ham_qlayer2qlayer(g["kpts"])
+    for ik in range(g["kpts"].shape[0]):
+        for iw, w in enumerate(g["omegas"]):
+            gll, grr, gb, it = gpu.surfgreen_1985(w, h00[ik], h01[ik], float(g["eta"]))
+            assert it == its[ik, iw]
+            for mine, ref in ((gll, g["diag_gll"]), (grr, g["diag_grr"]), (gb, g["diag_gb"])):
+                assert max_rel(np.diag(mine), ref[ik, iw]) < tol["diag"]
 
 
 def test_spectral_grid_ndim16_subgrid_vs_oracle(gpu, oracle_lib):
