@@ -24,7 +24,12 @@ typedef struct wtsurf_ctx wtsurf_ctx;
 /* Precision of the decimation. */
 enum {
     WTSURF_FP64 = 0,        /* all arithmetic in FP64, as the reference */
-    WTSURF_FP32_REFINE = 1  /* FP32 MFMA decimation + FP64 refinement */
+    WTSURF_FP32_REFINE = 1  /* FP32 MFMA inverse and GEMMs, FP64 accumulators, final
+                               inverses and traces: a speed/accuracy measurement mode.
+                               The doubling recursion amplifies FP32 rounding (relative
+                               errors ~1e-2 at Ndim 72, more on some instances), and
+                               beyond Ndim 80 it diverges on some instances, so runs
+                               with Ndim > 80 fail with an error in this mode. */
 };
 
 /* Per-instance status bits (wtsurf_grid_download / wtsurf_spectral_grid). */

@@ -307,3 +307,18 @@ def test_fp32_path_vs_fp64(gpu, golden, name):
     assert max(errs) < {"ndim16": 1e-3, "np3_ndim12": 1e-3, "ndim72": 5e-2}[name]
     assert np.all(np.abs(d32[3] - d64[3]) <= 1)
     assert np.all(d32[4] == 0)
+
+
+def test_fp32_path_refused_beyond_order_80(gpu, golden):
+    """FP32 working precision is refused beyond Ndim 80, where the doubling
+    recursion in FP32 diverges on some instances (on the Ndim 216 and 288
+    golden cases a numpy complex64 emulation overflows, and the device runs hit
+    itermax or settle on wrong values); the FP64 path runs as usual."""
+    _say("fp32 refused", "ndim216")
+    g = golden("ndim216")
+    nw, np_, irvec, hr = golden_model(g, "ndim216")
+    gpu.set_model_arrays(nw, np_, irvec, hr)
+    with pytest.raises(WtsurfError, match="FP32"):
+        gpu.spectral_grid(g["kpts"], g["omegas"], float(g["eta"]), precision="fp32_refine")
+    d64 = gpu.spectral_grid(g["kpts"], g["omegas"], float(g["eta"]), precision="fp64")
+    assert np.all(d64[4] == 0)

@@ -33,6 +33,7 @@
 // state in global memory between rounds and takes the next (k, omega)
 // instance from a device counter once its previous one is written out.
 #include "wtsurf_device.h"
+#include <type_traits>
 #include "wtsurf_internal.h"
 
 namespace wtsurf {
@@ -1422,7 +1423,9 @@ int decimate_max_q() { return kMaxQ; }
 
 template <int Q, typename R>
 static int launch_round(const DecimateArgs &a, int per_set, int set, bool gj, hipStream_t s) {
-    if constexpr (Q <= 5) {
+    if constexpr (Q > 5 && std::is_same_v<R, float>) {
+        return -1;  // no FP32 kernels beyond order 80 (the C ABI refuses that case)
+    } else if constexpr (Q <= 5) {
         constexpr size_t gj_lds = GjLay<Q, R>::bytes;
         if (gj)
             hipLaunchKernelGGL((gj_round_kernel<Q, R>), dim3(per_set), dim3(kGjBlock), gj_lds, s, a,
@@ -1476,9 +1479,11 @@ static void allow_lds() {
     if constexpr (Q <= 5)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(&gj_round_kernel<Q, R>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)GjLay<Q, R>::bytes);
-    else
+    else if constexpr (std::is_same_v<R, double>)
         e = hipFuncSetAttribute(reinterpret_cast<const void *>(&gj_large_round_kernel<Q, R>),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)GjLayL<Q>::bytes);
+    else
+        return;
     if (e != hipSuccess) (void)hipGetLastError();
 }
 

@@ -339,6 +339,9 @@ static int launch_decimation(wtsurf_ctx *c, int ninst, int nk, int nomega, const
                              int precision, InstanceLists inst = InstanceLists()) {
     const int q = padded_q(n);
     if (q > decimate_max_q()) return c->fail("Ndim exceeds the largest built decimation kernel");
+    if (precision == WTSURF_FP32_REFINE && q > 5)
+        return c->fail("WTSURF_FP32_REFINE is limited to Ndim <= 80: beyond that the doubling "
+                       "recursion in FP32 diverges on some instances (use WTSURF_FP64)");
     if (int r = ensure_work(c, q, precision, ninst)) return r;
     if (c->dos_l.ensure(ninst) || c->dos_r.ensure(ninst) || c->dos_b.ensure(ninst) ||
         c->iters.ensure(ninst) || c->status.ensure(ninst))
