@@ -277,7 +277,7 @@ __device__ __forceinline__ void gj_update_tile(typename Prec<R>::C (*S)[16 * Q +
 //                       keep their content (the p term is zero there),
 //   other rows:         S + M_p T_p + M_q T_q;
 // q's columns are left as q's elimination left them.
-// A tile's global-memory operands for gj_update_tile2: q's columns of S in the
+// A tile's global-memory operands for gj_tile2_finish: q's columns of S in the
 // tile's rows (the MFMA A operand) and the tile's old values.
 template <typename C>
 struct Tile2Ops {
