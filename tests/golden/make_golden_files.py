@@ -10,13 +10,16 @@ arc.dat_{l,r,bulk} of SurfaceDOSkk (SlabArc_calc) at the top; the QPI run
 spin texture spindos.dat_*) under qpi/ and the spin texture run
 (SlabSpintexture_calc) under spintexture/ and the k-path QPI run
 (SlabQPI_kpath_calc: surfstat_jdos's dos.dat_*, spindos.dat_*, dos.jdat_*)
-under qpi_kpath/, each with the wt.in it ran.
+under qpi_kpath/, each with the wt.in it ran.  The reference's own examples
+(EXAMPLES) run on their shipped wt.in and *_hr.dat, with only the grid sizes
+cut down.
 
     python tests/golden/make_golden_files.py
 """
 from __future__ import annotations
 
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -31,6 +34,7 @@ from wannier_tools_amd import model as M  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 WTX = os.path.join(ROOT, "oracle", "_ref", "wt.x")
+REFERENCE = os.environ.get("WT_REFERENCE", "/root/reference")
 OUTPUTS = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk",
            # surfstat's spin-resolved traces (written whatever SOC is)
            "spindos.dat_l", "spindos.dat_r"]
@@ -128,8 +132,9 @@ def run_reference(d: str) -> None:
     values.  A process of its own gives SurfaceDOSkk fresh (zero) memory."""
     text = open(os.path.join(d, "wt.in")).read()
     for flag, out in (("SlabSS_calc", "dos.dat_l"), ("SlabArc_calc", "arc.dat_l")):
-        one = text.replace("SlabSS_calc = T", f"SlabSS_calc = {'T' if flag == 'SlabSS_calc' else 'F'}")
-        one = one.replace("SlabArc_calc = T", f"SlabArc_calc = {'T' if flag == 'SlabArc_calc' else 'F'}")
+        one = text
+        for other in ("SlabSS_calc", "SlabArc_calc"):
+            one = re.sub(rf"{other}\s*=\s*T", f"{other} = {'T' if other == flag else 'F'}", one)
         with open(os.path.join(d, "wt.in"), "w") as f:
             f.write(one)
         run_wtx(d, out)
@@ -182,6 +187,28 @@ CASES = {
                              runs=["base"]),
 }
 
+# The reference's own examples: their wt.in and *_hr.dat as shipped, with only
+# the grid sizes (&PARAMETERS entries) cut down so the outputs stay small.
+EXAMPLES = {
+    # two-band Weyl semimetal (SOC = 1, one s projector): SURFACE rows (1 0 0)
+    # and (0 0 1), R3' found by the reference's search; one KPATH_SLAB line
+    # -X -> X; Nk1 = Nk2 = 201 and OmegaNum = 400 cut to 21, 21 and 40
+    "weyl3d": dict(src="examples/3DWeyl-model", wtin="wt.in-surfacestates", hr="Weyl3D_hr.dat",
+                   sizes={"OmegaNum": 40, "Nk1": 21, "Nk2": 21}),
+}
+
+
+def write_example(d: str, c: dict) -> str:
+    src = os.path.join(REFERENCE, c["src"])
+    text = open(os.path.join(src, c["wtin"])).read()
+    for key, val in c["sizes"].items():
+        text, n = re.subn(rf"^({key}\s*=\s*)\S+", rf"\g<1>{val}", text, flags=re.M)
+        assert n == 1, f"{key} not found once in {c['wtin']}"
+    with open(os.path.join(d, "wt.in"), "w") as f:
+        f.write(text)
+    shutil.copy(os.path.join(src, c["hr"]), os.path.join(d, c["hr"]))
+    return c["hr"]
+
 
 def main() -> None:
     subprocess.run(["make", "-s", "-j8", "-C", os.path.join(ROOT, "oracle"), "refexe"], check=True)
@@ -206,6 +233,16 @@ def main() -> None:
                 for fn in ["wt.in", f"{name}_hr.dat"] + outputs:
                     shutil.copy(os.path.join(d, fn), os.path.join(sub, fn))
             print(f"wrote {sub}")
+    for name, c in EXAMPLES.items():
+        dst = os.path.join(OUT, f"files_{name}")
+        shutil.rmtree(dst, ignore_errors=True)
+        with tempfile.TemporaryDirectory() as d:
+            hr = write_example(d, c)
+            run_reference(d)
+            os.makedirs(dst)
+            for fn in ["wt.in", hr] + OUTPUTS:
+                shutil.copy(os.path.join(d, fn), os.path.join(dst, fn))
+        print(f"wrote {dst}")
 
 
 if __name__ == "__main__":

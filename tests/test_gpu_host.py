@@ -19,7 +19,8 @@ from helpers import compare_tables
 
 pytestmark = pytest.mark.gpu
 
-CASES = ["ndim16", "soc_ndim30", "soc_rot_ndim20", "remap_soc_ndim20"]
+# weyl3d: the reference's own example (examples/3DWeyl-model) with its grid cut down
+CASES = ["ndim16", "soc_ndim30", "soc_rot_ndim20", "remap_soc_ndim20", "weyl3d"]
 FILES = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk",
          "spindos.dat_l", "spindos.dat_r"]
 WT_SURF = os.path.join(ROOT, "wannier_tools_amd", "bin", "wt_surf")
@@ -34,14 +35,15 @@ RUNS = [("ndim16", "qpi"), ("qpi_soc_ndim20", "qpi"), ("qpi_soc_ndim20", "spinte
 
 def run_case(exe, name, tmp_path, run=None):
     src = os.path.join(GOLDEN, f"files_{name}", run or "")
-    for fn in ("wt.in", f"{name}_hr.dat"):
+    inputs = ["wt.in"] + [fn for fn in os.listdir(src) if fn.endswith("_hr.dat")]
+    for fn in inputs:
         shutil.copy(os.path.join(src, fn), tmp_path / fn)
     print("run", os.path.basename(exe), name, run or "", flush=True)
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120,
                        env=dict(os.environ, OPENBLAS_NUM_THREADS="1"))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     files = FILES if run is None else sorted(
-        fn for fn in os.listdir(src) if fn not in ("wt.in", f"{name}_hr.dat"))
+        fn for fn in os.listdir(src) if fn not in inputs)
     worst = {}
     for fn in files:
         worst[fn] = compare_tables(str(tmp_path / fn), os.path.join(src, fn))

@@ -1,9 +1,14 @@
 """The surface-cell remap (wannier_tools_amd.cell: the SURFACE card's U, the
 new cell's atom positions, get_hmnr_cell, the surface frame) against the
-reference program's files for a three-atom spinful model whose SURFACE rows
-(1 1 0) and (0 0 1) leave the reference to search for R3'
-(tests/golden/files_remap_soc_ndim20, written by oracle/_ref/wt.x).  The
-remapped H(R) drives the oracle; every printed trace must match."""
+reference program's files (written by oracle/_ref/wt.x) for
+
+* remap_soc_ndim20: a three-atom spinful model whose SURFACE rows (1 1 0) and
+  (0 0 1) leave the reference to search for R3';
+* weyl3d: the reference's own example examples/3DWeyl-model (two-band Weyl
+  semimetal, SURFACE rows (1 0 0) and (0 0 1), so R3' = (0 -1 0) by the
+  search), its wt.in with only Nk1, Nk2 and OmegaNum cut down.
+
+The remapped H(R) drives the oracle; every printed trace must match."""
 import os
 
 import numpy as np
@@ -11,30 +16,51 @@ import pytest
 
 from conftest import GOLDEN
 from helpers import read_table
-from test_host_files import KPATH, orbitals
+from test_host_files import orbitals
 from wannier_tools_amd import cell as C
 from wannier_tools_amd import model as M
 
 NAME = "remap_soc_ndim20"
-CASE = dict(nw=10, np_=2, nprojs=[2, 1, 2], soc=True, nomega=11, nk1=7, nk2=5, eta_arc=0.005,
-            e_arc=0.1, efermi=0.0,
-            lattice=[[1.0, 0.1, 0.0], [0.2, 1.1, 0.0], [0.1, 0.2, 1.3]],
-            positions=[(0.1, 0.2, 0.3), (0.6, 0.55, 0.75), (0.35, 0.8, 0.1)],
-            surface=[[1, 1, 0], [0, 0, 1]])
+CASES = {
+    NAME: dict(hr=f"{NAME}_hr.dat", nw=10, np_=2, nprojs=[2, 1, 2], soc=True, nomega=11,
+               omega=(-1.0, 1.0), nk1=7, nk2=5, eta_arc=0.005, e_arc=0.1, efermi=0.0,
+               lattice=[[1.0, 0.1, 0.0], [0.2, 1.1, 0.0], [0.1, 0.2, 1.3]],
+               positions=[(0.1, 0.2, 0.3), (0.6, 0.55, 0.75), (0.35, 0.8, 0.1)],
+               surface=[[1, 1, 0], [0, 0, 1]],
+               kpath=[((0.0, 0.0), (0.5, 0.0)), ((0.5, 0.0), (0.5, 0.5))]),
+    "weyl3d": dict(hr="Weyl3D_hr.dat", nw=2, np_=2, nprojs=[1], soc=True, nomega=40,
+                   omega=(-1.6, 1.6), nk1=21, nk2=21, eta_arc=0.001, e_arc=0.0, efermi=0.0,
+                   lattice=[[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]],
+                   positions=[(0.0, 0.0, 0.0)], surface=[[1, 0, 0], [0, 0, 1]],
+                   kpath=[((-0.5, 0.0), (0.5, 0.0))]),
+}
+CASE = CASES[NAME]
 ANGSTROM2ATOMIC = 1.0 / 0.529177210903
 
 
-def remapped():
-    d = os.path.join(GOLDEN, f"files_{NAME}")
-    mdl = M.read_hr(os.path.join(d, f"{NAME}_hr.dat"), e_fermi=CASE["efermi"])
+def remapped(name=NAME):
+    c = CASES[name]
+    d = os.path.join(GOLDEN, f"files_{name}")
+    mdl = M.read_hr(os.path.join(d, c["hr"]), e_fermi=c["efermi"])
     hr = mdl.hr.astype(np.complex128, copy=True)
     zero = np.all(mdl.irvec == 0, axis=1)
     for ir in np.nonzero(zero)[0]:
         hr[ir][np.diag_indices(mdl.num_wann)] -= mdl.e_fermi
     hr *= M.EV2HARTREE
-    lat = np.array(CASE["lattice"]) * ANGSTROM2ATOMIC
-    return d, C.build_surface_cell(lat, np.array(CASE["positions"]), CASE["nprojs"], CASE["soc"],
-                                   mdl.irvec, hr, mdl.ndegen, CASE["surface"])
+    lat = np.array(c["lattice"]) * ANGSTROM2ATOMIC
+    return d, C.build_surface_cell(lat, np.array(c["positions"]), c["nprojs"], c["soc"],
+                                   mdl.irvec, hr, mdl.ndegen, c["surface"])
+
+
+def test_weyl_example_surface_matrix():
+    """U of the reference's Weyl example as its WT.out prints it ("New SURFACE
+    CARD": (1 0 0), (0 0 1), (0 -1 0)); the surface reciprocal vectors of the
+    unit cube are 2 pi / Angstrom along x and y of the surface frame."""
+    _, sc = remapped("weyl3d")
+    assert np.array_equal(sc.umatrix, [[1, 0, 0], [0, 0, 1], [0, -1, 0]])
+    two_pi = 2 * np.pi
+    assert np.allclose(sc.ka2 * ANGSTROM2ATOMIC, [two_pi, 0.0], atol=1e-12)
+    assert np.allclose(sc.kb2 * ANGSTROM2ATOMIC, [0.0, two_pi], atol=1e-12)
 
 
 def test_surface_matrix_search_and_positions():
@@ -61,14 +87,16 @@ def test_remap_preserves_every_hopping_once_per_target():
     assert np.allclose(got, vals, rtol=1e-14)
 
 
-def test_oracle_on_remapped_model_reproduces_reference_files(oracle_lib):
-    d, sc = remapped()
-    nw, np_ = CASE["nw"], CASE["np_"]
-    top = orbitals([1, 2, 3], CASE["nprojs"], 1)
-    lo, hi = -1.0 * M.EV2HARTREE, 1.0 * M.EV2HARTREE
-    om = M.omega_grid(lo, hi, CASE["nomega"])
-    eta = (hi - lo) / CASE["nomega"] * 3.0
-    kp = M.kpath_slab(KPATH, CASE["nk1"])
+@pytest.mark.parametrize("name", list(CASES))
+def test_oracle_on_remapped_model_reproduces_reference_files(oracle_lib, name):
+    d, sc = remapped(name)
+    c = CASES[name]
+    nw, np_ = c["nw"], c["np_"]
+    top = orbitals(list(range(1, len(c["nprojs"]) + 1)), c["nprojs"], 1)
+    lo, hi = c["omega"][0] * M.EV2HARTREE, c["omega"][1] * M.EV2HARTREE
+    om = M.omega_grid(lo, hi, c["nomega"])
+    eta = (hi - lo) / c["nomega"] * 3.0
+    kp = M.kpath_slab(c["kpath"], c["nk1"])
     dl, dr, db, _, sl, sr = oracle_lib.spectral_spin(nw, np_, sc.irvec, sc.hr, kp, om, eta, top, top,
                                                      spin_mode=1)
     for side, v in (("l", np.log(np.abs(dl))), ("r", np.log(np.abs(dr))),
@@ -79,9 +107,9 @@ def test_oracle_on_remapped_model_reproduces_reference_files(oracle_lib):
         ref = read_table(os.path.join(d, f"spindos.dat_{side}"))[1][:, 2:]
         got = np.einsum("ij,jkw->kwi", sc.urot, s).reshape(-1, 3)
         assert np.all(np.abs(got - ref) <= 2e-7 * np.abs(ref) + 2e-9 * np.abs(ref).max()), side
-    k12, nkx, nky = M.kplane_slab((-0.5, -0.5), (1.0, 0.0), (0.0, 1.0), CASE["nk1"], CASE["nk2"])
-    dl, dr, db, _ = oracle_lib.spectral(nw, np_, sc.irvec, sc.hr, k12, [CASE["e_arc"] * M.EV2HARTREE],
-                                        CASE["eta_arc"] * M.EV2HARTREE, top, top, divisor=M.REF_PI)
+    k12, nkx, nky = M.kplane_slab((-0.5, -0.5), (1.0, 0.0), (0.0, 1.0), c["nk1"], c["nk2"])
+    dl, dr, db, _ = oracle_lib.spectral(nw, np_, sc.irvec, sc.hr, k12, [c["e_arc"] * M.EV2HARTREE],
+                                        c["eta_arc"] * M.EV2HARTREE, top, top, divisor=M.REF_PI)
     for side, v in (("l", np.log(dl[:, 0])), ("r", np.log(dr[:, 0])), ("bulk", np.log(np.abs(db[:, 0])))):
         ref = read_table(os.path.join(d, f"arc.dat_{side}"))[1]
         assert np.allclose(v, ref[:, 2], rtol=0, atol=2e-8 + 2e-8 * np.abs(ref[:, 2]).max()), side
@@ -124,6 +152,8 @@ def read_dump(path):
 @pytest.mark.skipif(not os.path.exists(WT_SURF), reason="wt_surf not built")
 @pytest.mark.parametrize("name,lat,pos,nprojs,soc,surface,efermi", [
     (NAME, CASE["lattice"], CASE["positions"], CASE["nprojs"], True, CASE["surface"], 0.0),
+    ("weyl3d", CASES["weyl3d"]["lattice"], CASES["weyl3d"]["positions"], [1], True,
+     CASES["weyl3d"]["surface"], 0.0),
     ("soc_ndim30", [[1, 0, 0], [0, 1, 0], [0, 0, 1]], [(0, 0, 0), (0.1, 0.05, 0.2)], [3, 2], True,
      [[1, 0, 0], [0, 1, 0]], 0.0),
     ("ndim16", [[1.1, 0, 0], [0.3, 0.9, 0], [0, 0.1, 1.3]], [(0, 0, 0)], [8], False,
@@ -135,13 +165,14 @@ def test_fortran_host_remap_matches(tmp_path, name, lat, pos, nprojs, soc, surfa
     import shutil
     import subprocess
     d = os.path.join(GOLDEN, f"files_{name}")
-    for fn in ("wt.in", f"{name}_hr.dat"):
+    hrfile = CASES[name]["hr"] if name in CASES else f"{name}_hr.dat"
+    for fn in ("wt.in", hrfile):
         shutil.copy(os.path.join(d, fn), tmp_path / fn)
     r = subprocess.run([WT_SURF, "--dump-cell"], cwd=tmp_path, capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     got = read_dump(tmp_path / "surface_cell.dat")
-    mdl = M.read_hr(os.path.join(d, f"{name}_hr.dat"))
+    mdl = M.read_hr(os.path.join(d, hrfile))
     hr = mdl.hr.astype(np.complex128, copy=True)
     zero = np.all(mdl.irvec == 0, axis=1)
     for ir in np.nonzero(zero)[0]:
@@ -154,3 +185,54 @@ def test_fortran_host_remap_matches(tmp_path, name, lat, pos, nprojs, soc, surfa
     assert np.allclose(got["urot"], want.urot, atol=1e-15)
     assert np.allclose(got["ka2"], want.ka2, rtol=1e-14, atol=1e-15)
     assert np.allclose(got["kb2"], want.kb2, rtol=1e-14, atol=1e-15)
+
+
+def test_hr_reader_places_lines_by_their_indices(tmp_path):
+    """src/readHmnR.f90:101-124 places every line of *_hr.dat by its own
+    (R, m, n), so line order does not matter: the Weyl example lists n fastest
+    (m = 1 1 2 2, n = 1 2 1 2) where Wannier90 writes m fastest, and a shuffled
+    file reads the same as the ordered one."""
+    mdl = M.read_hr(os.path.join(GOLDEN, "files_weyl3d", "Weyl3D_hr.dat"))
+    ir = [tuple(r) for r in mdl.irvec].index((0, 1, 0))
+    # lines "0 1 0 1 2 0.5 0.0" and "0 1 0 2 1 -0.5 0.0"
+    assert mdl.hr[ir][0, 1] == 0.5 and mdl.hr[ir][1, 0] == -0.5
+    rng = np.random.default_rng(5)
+    nw, nr = 3, 4
+    irvec = np.array([[0, 0, 0], [1, 0, 0], [0, -1, 2], [-1, 1, 0]], dtype=np.int32)
+    hr = rng.standard_normal((nr, nw, nw)) + 1j * rng.standard_normal((nr, nw, nw))
+    M.write_hr(str(tmp_path / "a_hr.dat"), M.TBModel(nw, irvec, hr, np.ones(nr, dtype=np.int32)))
+    lines = open(tmp_path / "a_hr.dat").read().splitlines()
+    head, body = lines[:4], lines[4:]
+    body = [body[i] for i in rng.permutation(len(body))]
+    (tmp_path / "b_hr.dat").write_text("\n".join(head + body) + "\n")
+    for fn in ("a_hr.dat", "b_hr.dat"):
+        got = M.read_hr(str(tmp_path / fn))
+        where = {tuple(r): k for k, r in enumerate(got.irvec)}
+        for k, r in enumerate(irvec):
+            assert np.allclose(got.hr[where[tuple(r)]], hr[k], rtol=1e-15, atol=1e-15), fn
+
+
+@pytest.mark.skipif(not os.path.exists(WT_SURF), reason="wt_surf not built")
+def test_fortran_host_reads_hr_lines_by_their_indices(tmp_path):
+    """wt_surf's reader places lines by their own (R, m, n) too: the Weyl
+    example with its H(R) lines shuffled gives the same remapped model."""
+    import shutil
+    import subprocess
+    d = os.path.join(GOLDEN, "files_weyl3d")
+    lines = open(os.path.join(d, "Weyl3D_hr.dat")).read().splitlines()
+    head, body = lines[:4], lines[4:]
+    body = [body[i] for i in np.random.default_rng(3).permutation(len(body))]
+    dumps = []
+    for tag, text in (("ordered", "\n".join(lines)), ("shuffled", "\n".join(head + body))):
+        run = tmp_path / tag
+        run.mkdir()
+        shutil.copy(os.path.join(d, "wt.in"), run / "wt.in")
+        (run / "Weyl3D_hr.dat").write_text(text + "\n")
+        r = subprocess.run([WT_SURF, "--dump-cell"], cwd=run, capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 0, r.stdout + r.stderr
+        dumps.append(read_dump(run / "surface_cell.dat"))
+    _, sc = remapped("weyl3d")
+    for got in dumps:
+        assert np.array_equal(got["irvec"], sc.irvec)
+        assert np.allclose(got["hr"], sc.hr, rtol=1e-15, atol=1e-18)

@@ -713,7 +713,7 @@ contains
    !> E_fermi off the R = 0 diagonal, eV -> Hartree, then remapped onto the
    !> surface cell (get_hmnr_cell, with 1/ndegen applied there).
    subroutine read_hr()
-      integer :: u, ios, nw, nr, ir, m, n, i, r(3), mm, nn
+      integer :: u, ios, nw, nr, ir, n, i, r(3), mm, nn, nused
       real(dp) :: re, im
       integer, allocatable :: ndeg(:), irv(:, :)
       complex(dp), allocatable :: hmnr(:, :, :)
@@ -733,16 +733,44 @@ contains
       allocate(ndeg(nr), irv(3, nr), hmnr(nw, nw, nr))
       read(u, *) ndeg
       hmnr = (0d0, 0d0)
-      do ir = 1, nr
-         do n = 1, nw
-            do m = 1, nw
-               read(u, *) r, mm, nn, re, im
-               hmnr(mm, nn, ir) = dcmplx(re, im)
-            end do
+      !> every line is placed by its own R and (m, n), R vectors numbered in
+      !> order of first appearance (src/readHmnR.f90:101-124), so the order of
+      !> the lines does not matter; the search starts at the newest R
+      nused = 0
+      do i = 1, nr*nw*nw
+         read(u, *, iostat=ios) r, mm, nn, re, im
+         if (ios /= 0) then
+            write(*, '(2a)') ' >>> ERROR: short or malformed H(R) lines in ', trim(Hrfile)
+            stop 1
+         end if
+         ir = 0
+         do n = nused, 1, -1
+            if (all(irv(:, n) == r)) then
+               ir = n
+               exit
+            end if
          end do
-         irv(:, ir) = r
+         if (ir == 0) then
+            if (nused == nr) then
+               write(*, '(a, i0, 2a)') ' >>> ERROR: more than ', nr, ' R vectors in ', trim(Hrfile)
+               stop 1
+            end if
+            nused = nused + 1
+            ir = nused
+            irv(:, ir) = r
+         end if
+         if (mm < 1 .or. mm > nw .or. nn < 1 .or. nn > nw) then
+            write(*, '(2a)') ' >>> ERROR: orbital index out of range in ', trim(Hrfile)
+            stop 1
+         end if
+         hmnr(mm, nn, ir) = dcmplx(re, im)
       end do
       close(u)
+      if (nused /= nr) then
+         write(*, '(a, i0, a, i0, 2a)') ' >>> ERROR: ', nused, ' R vectors, header says ', nr, &
+            ' in ', trim(Hrfile)
+         stop 1
+      end if
       do ir = 1, nr
          if (all(irv(:, ir) == 0)) then
             do i = 1, nw

@@ -228,7 +228,10 @@ def write_hr(path: str, model: TBModel, header: str = "written by wannier_tools_
 
 
 def read_hr(path: str, e_fermi: float = 0.0) -> TBModel:
-    """Read a Wannier90 ``*_hr.dat`` (src/readHmnR.f90:30-120 reads the same layout)."""
+    """Read a Wannier90 ``*_hr.dat`` as src/readHmnR.f90:64-124 does: every line
+    ``R1 R2 R3 m n Re Im`` is placed by its own indices, so the order of the
+    lines does not matter; R vectors are numbered in order of first appearance
+    and ndegen is taken in that order."""
     with open(path) as f:
         f.readline()
         nw = int(f.readline().split()[0])
@@ -239,9 +242,19 @@ def read_hr(path: str, e_fermi: float = 0.0) -> TBModel:
         data = np.loadtxt(f, dtype=np.float64, ndmin=2)
     if data.shape[0] != nr * nw * nw:
         raise ValueError(f"{path}: expected {nr * nw * nw} H(R) lines, found {data.shape[0]}")
-    irvec = data[:: nw * nw, 0:3].astype(np.int32)
-    vals = (data[:, 5] + 1j * data[:, 6]).reshape(nr, nw, nw)  # [iR, n, m] with m fastest
-    hr = np.ascontiguousarray(vals.transpose(0, 2, 1))
+    rows = np.rint(data[:, 0:5]).astype(np.int64)
+    first: dict[tuple[int, int, int], int] = {}
+    for r in map(tuple, rows[:, 0:3]):
+        first.setdefault(r, len(first))
+    if len(first) != nr:
+        raise ValueError(f"{path}: {len(first)} distinct R vectors, header says {nr}")
+    m, n = rows[:, 3] - 1, rows[:, 4] - 1
+    if m.min() < 0 or n.min() < 0 or m.max() >= nw or n.max() >= nw:
+        raise ValueError(f"{path}: orbital index outside 1..{nw}")
+    ir = np.array([first[r] for r in map(tuple, rows[:, 0:3])], dtype=np.int64)
+    hr = np.zeros((nr, nw, nw), dtype=np.complex128)
+    hr[ir, m, n] = data[:, 5] + 1j * data[:, 6]
+    irvec = np.array(list(first), dtype=np.int32).reshape(nr, 3)
     return TBModel(num_wann=nw, irvec=irvec, hr=hr,
                    ndegen=np.array(nd[:nr], dtype=np.int32), e_fermi=e_fermi)
 
