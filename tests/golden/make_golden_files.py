@@ -125,13 +125,14 @@ def run_wtx(d: str, out: str) -> None:
         raise RuntimeError(f"reference wt.x failed in {d}:\n{r.stdout[-2000:]}\n{r.stderr[-2000:]}")
 
 
-def run_reference(d: str) -> None:
+def run_reference(d: str, drivers=("SlabSS_calc", "SlabArc_calc")) -> None:
     """One reference process per driver: SurfaceDOSkk accumulates dos_bulk into an
     array it never zeroes (src/fermiarc.f90:247 on the allocation of :77-82), so
     after surfstat has run in the same process arc.dat_bulk carries stale heap
     values.  A process of its own gives SurfaceDOSkk fresh (zero) memory."""
     text = open(os.path.join(d, "wt.in")).read()
-    for flag, out in (("SlabSS_calc", "dos.dat_l"), ("SlabArc_calc", "arc.dat_l")):
+    outs = {"SlabSS_calc": "dos.dat_l", "SlabArc_calc": "arc.dat_l"}
+    for flag, out in ((f, outs[f]) for f in drivers):
         one = text
         for other in ("SlabSS_calc", "SlabArc_calc"):
             one = re.sub(rf"{other}\s*=\s*T", f"{other} = {'T' if other == flag else 'F'}", one)
@@ -195,6 +196,15 @@ EXAMPLES = {
     # -X -> X; Nk1 = Nk2 = 201 and OmegaNum = 400 cut to 21, 21 and 40
     "weyl3d": dict(src="examples/3DWeyl-model", wtin="wt.in-surfacestates", hr="Weyl3D_hr.dat",
                    sizes={"OmegaNum": 40, "Nk1": 21, "Nk2": 21}),
+    # four-band BHZ model with a 0.5 eV Zeeman field along z (Add_Zeeman_Field,
+    # Bx, By, Bz on one line): SURFACE given as three rows (0 0 1), (1 0 0),
+    # (0 1 0), three KPATH_SLAB lines, NP and Eta_Arc left at their defaults.
+    # Its &CONTROL (bulk, slab-band, Hall and Wilson-loop drivers) is replaced
+    # by SlabSS_calc alone (the card set has no KPLANE_SLAB for SlabArc_calc);
+    # OmegaNum = 400 and Nk1 = Nk2 = 60 cut to 40, 20 and 10
+    "bhz_zeeman": dict(src="examples/BHZ-model", wtin="wt.in", hr="BHZ_hr.dat",
+                       sizes={"OmegaNum": 40, "Nk1": 20, "Nk2": 10},
+                       control="SlabSS_calc = T", drivers=("SlabSS_calc",)),
 }
 
 
@@ -204,6 +214,10 @@ def write_example(d: str, c: dict) -> str:
     for key, val in c["sizes"].items():
         text, n = re.subn(rf"^({key}\s*=\s*)\S+", rf"\g<1>{val}", text, flags=re.M)
         assert n == 1, f"{key} not found once in {c['wtin']}"
+    if "control" in c:
+        text, n = re.subn(r"&CONTROL\n.*?\n/", "&CONTROL\n" + c["control"] + "\n/", text,
+                          flags=re.S)
+        assert n == 1, f"no &CONTROL group in {c['wtin']}"
     with open(os.path.join(d, "wt.in"), "w") as f:
         f.write(text)
     shutil.copy(os.path.join(src, c["hr"]), os.path.join(d, c["hr"]))
@@ -238,9 +252,9 @@ def main() -> None:
         shutil.rmtree(dst, ignore_errors=True)
         with tempfile.TemporaryDirectory() as d:
             hr = write_example(d, c)
-            run_reference(d)
+            run_reference(d, c.get("drivers", ("SlabSS_calc", "SlabArc_calc")))
             os.makedirs(dst)
-            for fn in ["wt.in", hr] + OUTPUTS:
+            for fn in ["wt.in", hr] + [fn for fn in OUTPUTS if os.path.exists(os.path.join(d, fn))]:
                 shutil.copy(os.path.join(d, fn), os.path.join(dst, fn))
         print(f"wrote {dst}")
 

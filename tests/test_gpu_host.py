@@ -19,8 +19,9 @@ from helpers import compare_tables
 
 pytestmark = pytest.mark.gpu
 
-# weyl3d: the reference's own example (examples/3DWeyl-model) with its grid cut down
-CASES = ["ndim16", "soc_ndim30", "soc_rot_ndim20", "remap_soc_ndim20", "weyl3d"]
+# weyl3d and bhz_zeeman: the reference's own examples (examples/3DWeyl-model,
+# examples/BHZ-model with its Zeeman field) with their grids cut down
+CASES = ["ndim16", "soc_ndim30", "soc_rot_ndim20", "remap_soc_ndim20", "weyl3d", "bhz_zeeman"]
 FILES = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk",
          "spindos.dat_l", "spindos.dat_r"]
 WT_SURF = os.path.join(ROOT, "wannier_tools_amd", "bin", "wt_surf")
@@ -42,7 +43,7 @@ def run_case(exe, name, tmp_path, run=None):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120,
                        env=dict(os.environ, OPENBLAS_NUM_THREADS="1"))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    files = FILES if run is None else sorted(
+    files = [fn for fn in FILES if os.path.exists(os.path.join(src, fn))] if run is None else sorted(
         fn for fn in os.listdir(src) if fn not in inputs)
     worst = {}
     for fn in files:
@@ -55,7 +56,9 @@ def test_wt_surf_reproduces_reference_files(name, tmp_path):
     assert os.path.exists(WT_SURF), "wt_surf not built (run __graft_entry__.build())"
     run_case(WT_SURF, name, tmp_path)
     wt_out = open(tmp_path / "WT.out").read()
-    assert "Time cost for SlabSS_calc" in wt_out and "Time cost for SlabArc" in wt_out
+    assert "Time cost for SlabSS_calc" in wt_out
+    if os.path.exists(os.path.join(GOLDEN, f"files_{name}", "arc.dat_l")):
+        assert "Time cost for SlabArc" in wt_out
 
 
 @pytest.mark.skipif(not os.path.exists(WT_GPU), reason="reference program with the drop-in not built")

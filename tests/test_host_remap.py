@@ -6,7 +6,10 @@ reference program's files (written by oracle/_ref/wt.x) for
   (0 0 1) leave the reference to search for R3';
 * weyl3d: the reference's own example examples/3DWeyl-model (two-band Weyl
   semimetal, SURFACE rows (1 0 0) and (0 0 1), so R3' = (0 -1 0) by the
-  search), its wt.in with only Nk1, Nk2 and OmegaNum cut down.
+  search), its wt.in with only Nk1, Nk2 and OmegaNum cut down;
+* bhz_zeeman: the reference's examples/BHZ-model (four-band BHZ model, SURFACE
+  given as three rows, a 0.5 eV Zeeman field along z added to H(R = 0)), its
+  wt.in with the sizes cut down and &CONTROL set to SlabSS_calc alone.
 
 The remapped H(R) drives the oracle; every printed trace must match."""
 import os
@@ -33,6 +36,13 @@ CASES = {
                    lattice=[[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]],
                    positions=[(0.0, 0.0, 0.0)], surface=[[1, 0, 0], [0, 0, 1]],
                    kpath=[((-0.5, 0.0), (0.5, 0.0))]),
+    "bhz_zeeman": dict(hr="BHZ_hr.dat", nw=4, np_=2, nprojs=[2], soc=True, nomega=40,
+                       omega=(-8.0, 8.0), nk1=20, nk2=10, efermi=0.0,
+                       lattice=[[3.0, 0.0, 0.0], [0.0, 3.0, 0.0], [0.0, 0.0, 10.0]],
+                       positions=[(0.0, 0.0, 0.0)], surface=[[0, 0, 1], [1, 0, 0], [0, 1, 0]],
+                       kpath=[((0.0, -0.5), (0.0, 0.0)), ((0.0, 0.0), (0.0, 0.5)),
+                              ((0.0, 0.5), (0.5, 0.5))],
+                       zeeman=(0.5, (0.0, 0.0, 1.0))),
 }
 CASE = CASES[NAME]
 ANGSTROM2ATOMIC = 1.0 / 0.529177210903
@@ -47,6 +57,8 @@ def remapped(name=NAME):
     for ir in np.nonzero(zero)[0]:
         hr[ir][np.diag_indices(mdl.num_wann)] -= mdl.e_fermi
     hr *= M.EV2HARTREE
+    if "zeeman" in c:
+        hr = M.add_zeeman(mdl.irvec, hr, c["zeeman"][0] * M.EV2HARTREE, c["zeeman"][1])
     lat = np.array(c["lattice"]) * ANGSTROM2ATOMIC
     return d, C.build_surface_cell(lat, np.array(c["positions"]), c["nprojs"], c["soc"],
                                    mdl.irvec, hr, mdl.ndegen, c["surface"])
@@ -107,6 +119,8 @@ def test_oracle_on_remapped_model_reproduces_reference_files(oracle_lib, name):
         ref = read_table(os.path.join(d, f"spindos.dat_{side}"))[1][:, 2:]
         got = np.einsum("ij,jkw->kwi", sc.urot, s).reshape(-1, 3)
         assert np.all(np.abs(got - ref) <= 2e-7 * np.abs(ref) + 2e-9 * np.abs(ref).max()), side
+    if not os.path.exists(os.path.join(d, "arc.dat_l")):
+        return
     k12, nkx, nky = M.kplane_slab((-0.5, -0.5), (1.0, 0.0), (0.0, 1.0), c["nk1"], c["nk2"])
     dl, dr, db, _ = oracle_lib.spectral(nw, np_, sc.irvec, sc.hr, k12, [c["e_arc"] * M.EV2HARTREE],
                                         c["eta_arc"] * M.EV2HARTREE, top, top, divisor=M.REF_PI)
@@ -154,6 +168,8 @@ def read_dump(path):
     (NAME, CASE["lattice"], CASE["positions"], CASE["nprojs"], True, CASE["surface"], 0.0),
     ("weyl3d", CASES["weyl3d"]["lattice"], CASES["weyl3d"]["positions"], [1], True,
      CASES["weyl3d"]["surface"], 0.0),
+    ("bhz_zeeman", CASES["bhz_zeeman"]["lattice"], CASES["bhz_zeeman"]["positions"], [2], True,
+     CASES["bhz_zeeman"]["surface"], 0.0),
     ("soc_ndim30", [[1, 0, 0], [0, 1, 0], [0, 0, 1]], [(0, 0, 0), (0.1, 0.05, 0.2)], [3, 2], True,
      [[1, 0, 0], [0, 1, 0]], 0.0),
     ("ndim16", [[1.1, 0, 0], [0.3, 0.9, 0], [0, 0.1, 1.3]], [(0, 0, 0)], [8], False,
@@ -178,6 +194,9 @@ def test_fortran_host_remap_matches(tmp_path, name, lat, pos, nprojs, soc, surfa
     for ir in np.nonzero(zero)[0]:
         hr[ir][np.diag_indices(mdl.num_wann)] -= efermi
     hr *= M.EV2HARTREE
+    if "zeeman" in CASES.get(name, {}):
+        ez, bdir = CASES[name]["zeeman"]
+        hr = M.add_zeeman(mdl.irvec, hr, ez * M.EV2HARTREE, bdir)
     want = C.build_surface_cell(np.array(lat, float) * ANGSTROM2ATOMIC, np.array(pos, float), nprojs,
                                 soc, mdl.irvec, hr, mdl.ndegen, surface)
     assert np.array_equal(got["irvec"], want.irvec)
@@ -236,3 +255,30 @@ def test_fortran_host_reads_hr_lines_by_their_indices(tmp_path):
     for got in dumps:
         assert np.array_equal(got["irvec"], sc.irvec)
         assert np.allclose(got["hr"], sc.hr, rtol=1e-15, atol=1e-18)
+
+
+@pytest.mark.skipif(not os.path.exists(WT_SURF), reason="wt_surf not built")
+@pytest.mark.parametrize("setting,message", [
+    ("SOC = 0\nAdd_Zeeman_Field = T", "Add_Zeeman_Field with SOC = 0"),
+    ("SOC = 1\nElectric_field_in_eVpA = 0.1", "electric fields"),
+    ("SOC = 1\nSymmetrical_Electric_field_in_eVpA = 0.1", "electric fields"),
+    ("SOC = 1\nIs_Sparse_Hr = T", "dense Wannier90"),
+    ("SOC = 1\nOrthogonal_Basis = F", "dense Wannier90"),
+])
+def test_fortran_host_refuses_hamiltonian_terms_it_does_not_build(tmp_path, setting, message):
+    """Settings that change H(R) in ways the host does not implement stop the
+    run with a message instead of being ignored."""
+    import re
+    import shutil
+    import subprocess
+    d = os.path.join(GOLDEN, "files_bhz_zeeman")
+    text = open(os.path.join(d, "wt.in")).read()
+    text = re.sub(r"SOC = 1 .*\n", "", text)
+    text = re.sub(r"Add_Zeeman_Field = T\n", "", text)
+    text = text.replace("&SYSTEM\n", "&SYSTEM\n" + setting + "\n")
+    (tmp_path / "wt.in").write_text(text)
+    shutil.copy(os.path.join(d, "BHZ_hr.dat"), tmp_path / "BHZ_hr.dat")
+    r = subprocess.run([WT_SURF, "--dump-cell"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode != 0
+    assert message in r.stdout + r.stderr

@@ -13,7 +13,11 @@
 !   &CONTROL    SlabSS_calc, SlabArc_calc, SlabQPI_calc, SlabQPI_kplane_calc,
 !               SlabQPI_kpath_calc, SlabSpintexture_calc (other flags are accepted
 !               and ignored)
-!   &SYSTEM     SOC, E_FERMI
+!   &SYSTEM     SOC, E_FERMI; Add_Zeeman_Field with Zeeman_energy_in_eV or
+!               Effective_gfactor, Bmagnitude, Btheta, Bphi, Bx, By, Bz (SOC = 1);
+!               settings that change H(R) in ways not built here (Zeeman with
+!               SOC = 0, electric fields, sparse or non-orthogonal H(R)) stop
+!               the run with a message
 !   &PARAMETERS Eta_Arc, E_arc, OmegaNum, OmegaMin, OmegaMax, Nk1, Nk2, NP,
 !               topsurface_atom_index
 !   cards       LATTICE, ATOM_POSITIONS, PROJECTORS, SURFACE, KPATH_SLAB,
@@ -279,6 +283,11 @@ module wt_surf_input
    real(dp), allocatable :: cell_pos(:, :), cell_posn(:, :)
    integer, allocatable :: orb_atom(:)
 
+   ! Zeeman term added to H(R = 0) by read_hr (Add_Zeeman_Field): energy in
+   ! Hartree and the field's direction cosines
+   logical :: add_zeeman = .false.
+   real(dp) :: zeeman_au = 0d0, bdir(3) = 0d0
+
    integer, parameter :: maxkv = 512
    character(len=64) :: keys(maxkv)
    character(len=256) :: vals(maxkv)
@@ -538,6 +547,7 @@ contains
       call get_int('NP', Np)
       top_atom = 0
       call get_int('topsurface_atom_index', top_atom)
+      call read_fields()
       if (eta_arc > -998d0) Fermi_broadening = eta_arc
       if (e_arc > -998d0) iso_energy = e_arc
       iso_energy = iso_energy*eV2Hartree
@@ -709,6 +719,91 @@ contains
       end do
    end subroutine read_wt_in
 
+   !> The &SYSTEM / &TB_FILE settings that change H(R) before the remap.  The
+   !> Zeeman term with SOC = 1 is applied by read_hr; H(R) sources and terms
+   !> this host does not build (sparse or non-orthogonal H(R), the SOC = 0 spin
+   !> extension, electric fields) stop the run rather than being ignored.
+   subroutine read_fields()
+      logical :: flag, ortho, sparse, sparse_hr, hrfile
+      real(dp) :: ez, gfac, bmag, btheta, bphi, bx, by, bz, ef1, ef2
+      ! src/readinput.f90:47-50 and 66: the H(R) source
+      hrfile = .true.
+      sparse = .false.
+      sparse_hr = .false.
+      ortho = .true.
+      call get_logical('Is_HrFile', hrfile)
+      call get_logical('Is_Sparse', sparse)
+      call get_logical('Is_Sparse_Hr', sparse_hr)
+      call get_logical('Orthogonal_Basis', ortho)
+      if (.not. hrfile .or. sparse .or. sparse_hr .or. .not. ortho) then
+         write(*, '(a)') ' >>> ERROR: only a dense Wannier90 *_hr.dat in an orthogonal basis is read'
+         write(*, '(a)') '     here (Is_HrFile = T, Is_Sparse(_Hr) = F, Orthogonal_Basis = T)'
+         stop 1
+      end if
+      ! src/readinput.f90:355-369: field defaults
+      ef1 = 0d0
+      ef2 = 0d0
+      call get_real('Electric_field_in_eVpA', ef1)
+      call get_real('Symmetrical_Electric_field_in_eVpA', ef2)
+      if (ef1 /= 0d0 .or. ef2 /= 0d0) then
+         write(*, '(a)') ' >>> ERROR: electric fields on H(R) (Electric_field_in_eVpA,'
+         write(*, '(a)') '     Symmetrical_Electric_field_in_eVpA) are not supported here'
+         stop 1
+      end if
+      flag = .false.
+      call get_logical('Add_Zeeman_Field', flag)
+      if (.not. flag) return
+      if (Soc == 0) then
+         write(*, '(a)') ' >>> ERROR: Add_Zeeman_Field with SOC = 0 (spin-extended H(R)) is not'
+         write(*, '(a)') '     supported here; give the spinful H(R) with SOC = 1'
+         stop 1
+      end if
+      ez = 0d0
+      gfac = 2d0
+      bmag = 0d0
+      btheta = -99999d0
+      bphi = -99999d0
+      bx = 0d0
+      by = 0d0
+      bz = 0d0
+      call get_real('Zeeman_energy_in_eV', ez)
+      call get_real('Effective_gfactor', gfac)
+      call get_real('Bmagnitude', bmag)
+      call get_real('Btheta', btheta)
+      call get_real('Bphi', bphi)
+      call get_real('Bx', bx)
+      call get_real('By', by)
+      call get_real('Bz', bz)
+      ! the field's direction and magnitude in Tesla: Bmagnitude with Btheta and
+      ! Bphi first, else Bx, By, Bz, else Btheta and Bphi alone with magnitude 1
+      ! (src/readinput.f90:464-565)
+      if (abs(bmag) > eps6) then
+         if (abs(btheta + 99999d0) < eps6) btheta = 0d0
+         if (abs(bphi + 99999d0) < eps6) bphi = 0d0
+         bdir = (/sin(btheta/180d0*pi)*cos(bphi/180d0*pi), sin(btheta/180d0*pi)*sin(bphi/180d0*pi), &
+                  cos(btheta/180d0*pi)/)
+      else if (abs(bx) + abs(by) + abs(bz) > eps6) then
+         bmag = sqrt(bx*bx + by*by + bz*bz)
+         bdir = (/bx, by, bz/)/bmag
+      else
+         if (abs(btheta + 99999d0) < eps6) btheta = 0d0
+         if (abs(bphi + 99999d0) < eps6) bphi = 0d0
+         bdir = (/sin(btheta/180d0*pi)*cos(bphi/180d0*pi), sin(btheta/180d0*pi)*sin(bphi/180d0*pi), &
+                  cos(btheta/180d0*pi)/)
+         bmag = 1d0
+      end if
+      ! Tesla -> atomic units (src/readinput.f90:574-577, constants of
+      ! src/module.f90:710-715 as single-precision literals), then the Zeeman
+      ! energy g mu_B B unless given in eV (src/readHmnR.f90:1459-1464)
+      bmag = bmag*real(1.602176634e-19, dp)*real(5.29177210903e-11, dp)**2/real(1.054571817e-34, dp)
+      if (abs(ez) < eps9) then
+         zeeman_au = gfac*0.5d0*bmag
+      else
+         zeeman_au = ez*eV2Hartree
+      end if
+      add_zeeman = .true.
+   end subroutine read_fields
+
    !> Wannier90 *_hr.dat (src/readHmnR.f90:3-305 for the orthogonal basis):
    !> E_fermi off the R = 0 diagonal, eV -> Hartree, then remapped onto the
    !> surface cell (get_hmnr_cell, with 1/ndegen applied there).
@@ -782,6 +877,26 @@ contains
          hmnr = hmnr*eV2Hartree*eV2Hartree
       else
          hmnr = hmnr*eV2Hartree
+      end if
+      ! Zeeman term (B . sigma) E_Z / 2 on the R = 0 block, spin-up orbitals
+      ! 1..nw/2 paired with nw/2+1..nw (src/readHmnR.f90:1475-1484, SOC = 1)
+      if (add_zeeman) then
+         ir = 0
+         do n = 1, nr
+            if (all(irv(:, n) == 0)) ir = n
+         end do
+         if (ir == 0) then
+            write(*, '(a)') ' >>> ERROR: Add_Zeeman_Field needs an R = 0 block in the Hrfile'
+            stop 1
+         end if
+         do n = 1, nw/2
+            hmnr(n, n, ir) = hmnr(n, n, ir) + zeeman_au/2d0*bdir(3)
+            hmnr(n + nw/2, n + nw/2, ir) = hmnr(n + nw/2, n + nw/2, ir) - zeeman_au/2d0*bdir(3)
+            hmnr(n, n + nw/2, ir) = hmnr(n, n + nw/2, ir) + zeeman_au/2d0*bdir(1) &
+               - (0d0, 1d0)*zeeman_au/2d0*bdir(2)
+            hmnr(n + nw/2, n, ir) = hmnr(n + nw/2, n, ir) + zeeman_au/2d0*bdir(1) &
+               + (0d0, 1d0)*zeeman_au/2d0*bdir(2)
+         end do
       end if
       call remap_hr(cell_u, orb_atom, cell_pos, cell_posn, nr, irv, hmnr, ndeg)
    end subroutine read_hr

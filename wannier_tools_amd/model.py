@@ -4,7 +4,10 @@ Host-side data handling that mirrors the reference's model input:
 
 * ``read_hr`` / ``write_hr`` -- the Wannier90 ``*_hr.dat`` format read by
   ``readNormalHmnR`` (src/readHmnR.f90:3-305): header line, num_wann, nrpts,
-  ndegen (15 per line), then ``R1 R2 R3 m n Re Im`` with m fastest.
+  ndegen (15 per line), then ``R1 R2 R3 m n Re Im`` lines, each placed by its
+  own indices (``write_hr`` writes m fastest).
+* ``add_zeeman`` -- the Zeeman term of ``Add_Zeeman_Field`` on H(R = 0) for a
+  spinful model (src/readHmnR.f90:1431-1502).
 * ``TBModel.surface_cell`` -- what ``get_hmnr_cell`` (src/readHmnR.f90:578-766)
   leaves in ``HmnR_surfacecell`` for SURFACE = identity: H(R)/ndegen in
   Hartree with E_fermi taken off the R=0 diagonal (src/readHmnR.f90:140-159),
@@ -257,6 +260,27 @@ def read_hr(path: str, e_fermi: float = 0.0) -> TBModel:
     irvec = np.array(list(first), dtype=np.int32).reshape(nr, 3)
     return TBModel(num_wann=nw, irvec=irvec, hr=hr,
                    ndegen=np.array(nd[:nr], dtype=np.int32), e_fermi=e_fermi)
+
+
+def add_zeeman(irvec: np.ndarray, hr: np.ndarray, energy: float, direction) -> np.ndarray:
+    """H(R) with the Zeeman term (B . sigma) E_Z / 2 added to the R = 0 block
+    for a spinful model (orbitals 1..nw/2 spin up, nw/2+1..nw their spin-down
+    partners), as add_zeeman_normal_hr does for SOC = 1
+    (src/readHmnR.f90:1475-1484).  energy is E_Z in the units of hr; direction
+    the field's direction cosines.  Returns a new array."""
+    out = np.array(hr, dtype=np.complex128, copy=True)
+    zero = np.nonzero(np.all(np.asarray(irvec) == 0, axis=1))[0]
+    if zero.size == 0:
+        raise ValueError("the Zeeman term needs an R = 0 block")
+    ir0 = int(zero[-1])
+    h = out.shape[1] // 2
+    bx, by, bz = (float(x) for x in direction)
+    n = np.arange(h)
+    out[ir0, n, n] += energy / 2 * bz
+    out[ir0, n + h, n + h] -= energy / 2 * bz
+    out[ir0, n, n + h] += energy / 2 * bx - 1j * energy / 2 * by
+    out[ir0, n + h, n] += energy / 2 * bx + 1j * energy / 2 * by
+    return out
 
 
 def default_top_orbitals(num_wann: int) -> np.ndarray:
