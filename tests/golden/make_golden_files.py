@@ -205,6 +205,14 @@ EXAMPLES = {
     "bhz_zeeman": dict(src="examples/BHZ-model", wtin="wt.in", hr="BHZ_hr.dat",
                        sizes={"OmegaNum": 40, "Nk1": 20, "Nk2": 10},
                        control="SlabSS_calc = T", drivers=("SlabSS_calc",)),
+    # the same with the field given in Tesla instead of as an energy:
+    # Zeeman_energy_in_eV and Bx, By, Bz replaced by Bmagnitude = 2000 T at
+    # Btheta = 60, Bphi = 30 degrees and a g-factor of 2.5 (E_Z about 0.29 eV)
+    "bhz_zeeman_tesla": dict(src="examples/BHZ-model", wtin="wt.in", hr="BHZ_hr.dat",
+                             sizes={"OmegaNum": 40, "Nk1": 20, "Nk2": 10},
+                             control="SlabSS_calc = T", drivers=("SlabSS_calc",),
+                             edits=[(r"^Bx=.*$", "Bmagnitude = 2000, Btheta = 60, Bphi = 30"),
+                                    (r"^Zeeman_energy_in_eV.*$", "Effective_gfactor = 2.5")]),
 }
 
 
@@ -214,6 +222,9 @@ def write_example(d: str, c: dict) -> str:
     for key, val in c["sizes"].items():
         text, n = re.subn(rf"^({key}\s*=\s*)\S+", rf"\g<1>{val}", text, flags=re.M)
         assert n == 1, f"{key} not found once in {c['wtin']}"
+    for pattern, repl in c.get("edits", []):
+        text, n = re.subn(pattern, repl, text, flags=re.M)
+        assert n == 1, f"{pattern} not found once in {c['wtin']}"
     if "control" in c:
         text, n = re.subn(r"&CONTROL\n.*?\n/", "&CONTROL\n" + c["control"] + "\n/", text,
                           flags=re.S)

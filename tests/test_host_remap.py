@@ -9,7 +9,9 @@ reference program's files (written by oracle/_ref/wt.x) for
   search), its wt.in with only Nk1, Nk2 and OmegaNum cut down;
 * bhz_zeeman: the reference's examples/BHZ-model (four-band BHZ model, SURFACE
   given as three rows, a 0.5 eV Zeeman field along z added to H(R = 0)), its
-  wt.in with the sizes cut down and &CONTROL set to SlabSS_calc alone.
+  wt.in with the sizes cut down and &CONTROL set to SlabSS_calc alone;
+  bhz_zeeman_tesla the same with the field given in Tesla (2000 T at
+  theta = 60, phi = 30 degrees, g = 2.5).
 
 The remapped H(R) drives the oracle; every printed trace must match."""
 import os
@@ -42,8 +44,11 @@ CASES = {
                        positions=[(0.0, 0.0, 0.0)], surface=[[0, 0, 1], [1, 0, 0], [0, 1, 0]],
                        kpath=[((0.0, -0.5), (0.0, 0.0)), ((0.0, 0.0), (0.0, 0.5)),
                               ((0.0, 0.5), (0.5, 0.5))],
-                       zeeman=(0.5, (0.0, 0.0, 1.0))),
+                       zeeman=dict(zeeman_energy_ev=0.5, bx=0.0, by=0.0, bz=1.0)),
 }
+# the field in Tesla (Bmagnitude, Btheta, Bphi, g = 2.5) instead of an energy
+CASES["bhz_zeeman_tesla"] = dict(CASES["bhz_zeeman"], zeeman=dict(bmagnitude=2000.0, btheta=60.0,
+                                                                  bphi=30.0, gfactor=2.5))
 CASE = CASES[NAME]
 ANGSTROM2ATOMIC = 1.0 / 0.529177210903
 
@@ -58,7 +63,7 @@ def remapped(name=NAME):
         hr[ir][np.diag_indices(mdl.num_wann)] -= mdl.e_fermi
     hr *= M.EV2HARTREE
     if "zeeman" in c:
-        hr = M.add_zeeman(mdl.irvec, hr, c["zeeman"][0] * M.EV2HARTREE, c["zeeman"][1])
+        hr = M.add_zeeman(mdl.irvec, hr, *M.zeeman_field(**c["zeeman"]))
     lat = np.array(c["lattice"]) * ANGSTROM2ATOMIC
     return d, C.build_surface_cell(lat, np.array(c["positions"]), c["nprojs"], c["soc"],
                                    mdl.irvec, hr, mdl.ndegen, c["surface"])
@@ -170,6 +175,8 @@ def read_dump(path):
      CASES["weyl3d"]["surface"], 0.0),
     ("bhz_zeeman", CASES["bhz_zeeman"]["lattice"], CASES["bhz_zeeman"]["positions"], [2], True,
      CASES["bhz_zeeman"]["surface"], 0.0),
+    ("bhz_zeeman_tesla", CASES["bhz_zeeman"]["lattice"], CASES["bhz_zeeman"]["positions"], [2],
+     True, CASES["bhz_zeeman"]["surface"], 0.0),
     ("soc_ndim30", [[1, 0, 0], [0, 1, 0], [0, 0, 1]], [(0, 0, 0), (0.1, 0.05, 0.2)], [3, 2], True,
      [[1, 0, 0], [0, 1, 0]], 0.0),
     ("ndim16", [[1.1, 0, 0], [0.3, 0.9, 0], [0, 0.1, 1.3]], [(0, 0, 0)], [8], False,
@@ -195,8 +202,7 @@ def test_fortran_host_remap_matches(tmp_path, name, lat, pos, nprojs, soc, surfa
         hr[ir][np.diag_indices(mdl.num_wann)] -= efermi
     hr *= M.EV2HARTREE
     if "zeeman" in CASES.get(name, {}):
-        ez, bdir = CASES[name]["zeeman"]
-        hr = M.add_zeeman(mdl.irvec, hr, ez * M.EV2HARTREE, bdir)
+        hr = M.add_zeeman(mdl.irvec, hr, *M.zeeman_field(**CASES[name]["zeeman"]))
     want = C.build_surface_cell(np.array(lat, float) * ANGSTROM2ATOMIC, np.array(pos, float), nprojs,
                                 soc, mdl.irvec, hr, mdl.ndegen, surface)
     assert np.array_equal(got["irvec"], want.irvec)

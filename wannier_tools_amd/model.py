@@ -6,8 +6,9 @@ Host-side data handling that mirrors the reference's model input:
   ``readNormalHmnR`` (src/readHmnR.f90:3-305): header line, num_wann, nrpts,
   ndegen (15 per line), then ``R1 R2 R3 m n Re Im`` lines, each placed by its
   own indices (``write_hr`` writes m fastest).
-* ``add_zeeman`` -- the Zeeman term of ``Add_Zeeman_Field`` on H(R = 0) for a
-  spinful model (src/readHmnR.f90:1431-1502).
+* ``zeeman_field`` / ``add_zeeman`` -- the Zeeman term of ``Add_Zeeman_Field``
+  on H(R = 0) for a spinful model (src/readinput.f90:464-577,
+  src/readHmnR.f90:1431-1502).
 * ``TBModel.surface_cell`` -- what ``get_hmnr_cell`` (src/readHmnR.f90:578-766)
   leaves in ``HmnR_surfacecell`` for SURFACE = identity: H(R)/ndegen in
   Hartree with E_fermi taken off the R=0 diagonal (src/readHmnR.f90:140-159),
@@ -260,6 +261,44 @@ def read_hr(path: str, e_fermi: float = 0.0) -> TBModel:
     irvec = np.array(list(first), dtype=np.int32).reshape(nr, 3)
     return TBModel(num_wann=nw, irvec=irvec, hr=hr,
                    ndegen=np.array(nd[:nr], dtype=np.int32), e_fermi=e_fermi)
+
+
+# SI constants as src/module.f90:710-715 holds them (single-precision literals)
+_ECHARGE = float(np.float32(1.602176634e-19))
+_BOHR_RADIUS = float(np.float32(5.29177210903e-11))
+_HBAR = float(np.float32(1.054571817e-34))
+
+
+def zeeman_field(zeeman_energy_ev: float = 0.0, gfactor: float = 2.0, bmagnitude: float = 0.0,
+                 btheta: float = -99999.0, bphi: float = -99999.0, bx: float = 0.0,
+                 by: float = 0.0, bz: float = 0.0) -> tuple[float, np.ndarray]:
+    """(E_Z in Hartree, field direction cosines) from the &SYSTEM settings of
+    Add_Zeeman_Field (defaults as src/readinput.f90:355-365): the direction from
+    Bmagnitude with Btheta/Bphi, else from Bx, By, Bz, else from Btheta/Bphi with
+    magnitude 1 T (src/readinput.f90:464-565); B converted to atomic units
+    (:574-577); E_Z = g mu_B B unless Zeeman_energy_in_eV is given
+    (src/readHmnR.f90:1459-1464)."""
+    eps6, eps9 = float(np.float32(1e-6)), EPS9
+    deg = REF_PI / 180.0
+
+    def angles(t, p):
+        t = 0.0 if abs(t + 99999.0) < eps6 else t
+        p = 0.0 if abs(p + 99999.0) < eps6 else p
+        return np.array([np.sin(t * deg) * np.cos(p * deg), np.sin(t * deg) * np.sin(p * deg),
+                         np.cos(t * deg)])
+
+    if abs(bmagnitude) > eps6:
+        direction = angles(btheta, bphi)
+    elif abs(bx) + abs(by) + abs(bz) > eps6:
+        bmagnitude = float(np.sqrt(bx * bx + by * by + bz * bz))
+        direction = np.array([bx, by, bz]) / bmagnitude
+    else:
+        direction = angles(btheta, bphi)
+        bmagnitude = 1.0
+    b_au = bmagnitude * _ECHARGE * _BOHR_RADIUS ** 2 / _HBAR
+    if abs(zeeman_energy_ev) < eps9:
+        return gfactor * 0.5 * b_au, direction
+    return zeeman_energy_ev * EV2HARTREE, direction
 
 
 def add_zeeman(irvec: np.ndarray, hr: np.ndarray, energy: float, direction) -> np.ndarray:
