@@ -58,7 +58,7 @@ OmegaMax = 1.0
 Nk1 = {nk1}
 Nk2 = {nk2}
 NP = {np}
-/
+{params}/
 LATTICE
 Angstrom
 {a1}
@@ -102,7 +102,7 @@ def write_case(d: str, name: str, mdl: M.TBModel, soc: int, nprojs: list[int], n
                eta: float, earc: float, efermi: float, nomega: int, nk1: int, nk2: int,
                lattice, surface_atoms: str = "", control: str = RUNS["base"][0],
                kplane: str = KPLANE_DEFAULT, surface: str = "1 0 0\n0 1 0",
-               positions=None) -> None:
+               positions=None, params: str = "") -> None:
     hrname = f"{name}_hr.dat"
     M.write_hr(os.path.join(d, hrname), mdl)
     if positions is None:
@@ -113,7 +113,7 @@ def write_case(d: str, name: str, mdl: M.TBModel, soc: int, nprojs: list[int], n
                         hr=hrname, soc=soc, efermi=efermi, eta=eta, earc=earc, nomega=nomega,
                         nk1=nk1, nk2=nk2, np=np_, a1=lattice[0], a2=lattice[1], a3=lattice[2],
                         natoms=len(nprojs), atoms=atoms, nprojs=" ".join(map(str, nprojs)),
-                        projlines=projlines, surface_atoms=surface_atoms)
+                        projlines=projlines, surface_atoms=surface_atoms, params=params)
     with open(os.path.join(d, "wt.in"), "w") as f:
         f.write(text)
 
@@ -186,6 +186,17 @@ CASES = {
                              positions=[(0.1, 0.2, 0.3), (0.6, 0.55, 0.75), (0.35, 0.8, 0.1)],
                              surface="1 1 0\n0 0 1",
                              runs=["base"]),
+    # the same cell with topsurface_atom_index = 2: the new cell's atoms are
+    # shifted so that atom 2 (not the highest one) sits at 0.95 along R3'
+    # (src/readinput.f90:1761-1772), which moves hoppings to other R'
+    "remap_top2_soc_ndim20": dict(mdl=lambda: M.synthetic_model(10, 1, 0.35, M.decay_l1, seed=14,
+                                                                soc_sigma=0.06, name="remap10"),
+                                  soc=1, nprojs=[2, 1, 2], np_=2, eta=0.005, earc=0.1, efermi=0.0,
+                                  nomega=11, nk1=7, nk2=5,
+                                  lattice=["1.0 0.1 0.0", "0.2 1.1 0.0", "0.1 0.2 1.3"],
+                                  positions=[(0.1, 0.2, 0.3), (0.6, 0.55, 0.75), (0.35, 0.8, 0.1)],
+                                  surface="1 1 0\n0 0 1", params="topsurface_atom_index = 2\n",
+                                  runs=["base"]),
 }
 
 # The reference's own examples: their wt.in and *_hr.dat as shipped, with only
@@ -247,7 +258,8 @@ def main() -> None:
                 write_case(d, name, c["mdl"](), c["soc"], c["nprojs"], c["np_"], c["eta"],
                            c["earc"], c["efermi"], c["nomega"], c["nk1"], c["nk2"], c["lattice"],
                            c.get("surface_atoms", ""), control, c.get("kplane", KPLANE_DEFAULT),
-                           c.get("surface", "1 0 0\n0 1 0"), c.get("positions"))
+                           c.get("surface", "1 0 0\n0 1 0"), c.get("positions"),
+                           c.get("params", ""))
                 if run == "base":
                     run_reference(d)
                     outputs = OUTPUTS

@@ -11,7 +11,8 @@ reference program's files (written by oracle/_ref/wt.x) for
   given as three rows, a 0.5 eV Zeeman field along z added to H(R = 0)), its
   wt.in with the sizes cut down and &CONTROL set to SlabSS_calc alone;
   bhz_zeeman_tesla the same with the field given in Tesla (2000 T at
-  theta = 60, phi = 30 degrees, g = 2.5).
+  theta = 60, phi = 30 degrees, g = 2.5);
+* remap_top2_soc_ndim20: the three-atom cell with topsurface_atom_index = 2.
 
 The remapped H(R) drives the oracle; every printed trace must match."""
 import os
@@ -46,6 +47,9 @@ CASES = {
                               ((0.0, 0.5), (0.5, 0.5))],
                        zeeman=dict(zeeman_energy_ev=0.5, bx=0.0, by=0.0, bz=1.0)),
 }
+# topsurface_atom_index = 2 on the three-atom cell: atom 2, not the highest,
+# is shifted to 0.95 along R3'
+CASES["remap_top2_soc_ndim20"] = dict(CASES[NAME], hr="remap_top2_soc_ndim20_hr.dat", top_atom=2)
 # the field in Tesla (Bmagnitude, Btheta, Bphi, g = 2.5) instead of an energy
 CASES["bhz_zeeman_tesla"] = dict(CASES["bhz_zeeman"], zeeman=dict(bmagnitude=2000.0, btheta=60.0,
                                                                   bphi=30.0, gfactor=2.5))
@@ -66,7 +70,7 @@ def remapped(name=NAME):
         hr = M.add_zeeman(mdl.irvec, hr, *M.zeeman_field(**c["zeeman"]))
     lat = np.array(c["lattice"]) * ANGSTROM2ATOMIC
     return d, C.build_surface_cell(lat, np.array(c["positions"]), c["nprojs"], c["soc"],
-                                   mdl.irvec, hr, mdl.ndegen, c["surface"])
+                                   mdl.irvec, hr, mdl.ndegen, c["surface"], c.get("top_atom", 0))
 
 
 def test_weyl_example_surface_matrix():
@@ -177,6 +181,8 @@ def read_dump(path):
      CASES["bhz_zeeman"]["surface"], 0.0),
     ("bhz_zeeman_tesla", CASES["bhz_zeeman"]["lattice"], CASES["bhz_zeeman"]["positions"], [2],
      True, CASES["bhz_zeeman"]["surface"], 0.0),
+    ("remap_top2_soc_ndim20", CASE["lattice"], CASE["positions"], CASE["nprojs"], True,
+     CASE["surface"], 0.0),
     ("soc_ndim30", [[1, 0, 0], [0, 1, 0], [0, 0, 1]], [(0, 0, 0), (0.1, 0.05, 0.2)], [3, 2], True,
      [[1, 0, 0], [0, 1, 0]], 0.0),
     ("ndim16", [[1.1, 0, 0], [0.3, 0.9, 0], [0, 0.1, 1.3]], [(0, 0, 0)], [8], False,
@@ -204,7 +210,8 @@ def test_fortran_host_remap_matches(tmp_path, name, lat, pos, nprojs, soc, surfa
     if "zeeman" in CASES.get(name, {}):
         hr = M.add_zeeman(mdl.irvec, hr, *M.zeeman_field(**CASES[name]["zeeman"]))
     want = C.build_surface_cell(np.array(lat, float) * ANGSTROM2ATOMIC, np.array(pos, float), nprojs,
-                                soc, mdl.irvec, hr, mdl.ndegen, surface)
+                                soc, mdl.irvec, hr, mdl.ndegen, surface,
+                                CASES.get(name, {}).get("top_atom", 0))
     assert np.array_equal(got["irvec"], want.irvec)
     assert np.allclose(got["hr"], want.hr, rtol=1e-15, atol=1e-18)
     assert np.allclose(got["urot"], want.urot, atol=1e-15)
