@@ -197,6 +197,19 @@ CASES = {
                                   positions=[(0.1, 0.2, 0.3), (0.6, 0.55, 0.75), (0.35, 0.8, 0.1)],
                                   surface="1 1 0\n0 0 1", params="topsurface_atom_index = 2\n",
                                   runs=["base"]),
+    # the three-atom cell with its lattice in Bohr and the atoms in Cartesian
+    # coordinates (Bohr), src/readinput.f90:772-787, 871-933
+    "remap_cart_bohr_soc_ndim20": dict(mdl=lambda: M.synthetic_model(10, 1, 0.35, M.decay_l1,
+                                                                     seed=15, soc_sigma=0.06,
+                                                                     name="remapcart10"),
+                                       soc=1, nprojs=[2, 1, 2], np_=2, eta=0.005, earc=0.1,
+                                       efermi=0.0, nomega=11, nk1=7, nk2=5,
+                                       lattice=["1.0 0.1 0.0", "0.2 1.1 0.0", "0.1 0.2 1.3"],
+                                       positions=[(0.0, 1.0, 0.7), (0.4, 1.0, 0.4),
+                                                  (0.6, 0.2, 0.5)],
+                                       surface="1 1 0\n0 0 1",
+                                       edits=[(r"^Angstrom$", "Bohr"), (r"^Direct$", "Cartesian")],
+                                       runs=["base"]),
 }
 
 # The reference's own examples: their wt.in and *_hr.dat as shipped, with only
@@ -225,6 +238,16 @@ EXAMPLES = {
                              edits=[(r"^Bx=.*$", "Bmagnitude = 2000, Btheta = 60, Bphi = 30"),
                                     (r"^Zeeman_energy_in_eV.*$", "Effective_gfactor = 2.5")]),
 }
+
+
+def edit_wt_in(d: str, edits) -> None:
+    path = os.path.join(d, "wt.in")
+    text = open(path).read()
+    for pattern, repl in edits:
+        text, n = re.subn(pattern, repl, text, flags=re.M)
+        assert n == 1, f"{pattern} not found once in wt.in"
+    with open(path, "w") as f:
+        f.write(text)
 
 
 def write_example(d: str, c: dict) -> str:
@@ -260,6 +283,7 @@ def main() -> None:
                            c.get("surface_atoms", ""), control, c.get("kplane", KPLANE_DEFAULT),
                            c.get("surface", "1 0 0\n0 1 0"), c.get("positions"),
                            c.get("params", ""))
+                edit_wt_in(d, c.get("edits", []))
                 if run == "base":
                     run_reference(d)
                     outputs = OUTPUTS

@@ -12,7 +12,9 @@ reference program's files (written by oracle/_ref/wt.x) for
   wt.in with the sizes cut down and &CONTROL set to SlabSS_calc alone;
   bhz_zeeman_tesla the same with the field given in Tesla (2000 T at
   theta = 60, phi = 30 degrees, g = 2.5);
-* remap_top2_soc_ndim20: the three-atom cell with topsurface_atom_index = 2.
+* remap_top2_soc_ndim20: the three-atom cell with topsurface_atom_index = 2;
+* remap_cart_bohr_soc_ndim20: a three-atom cell with its LATTICE in Bohr and
+  ATOM_POSITIONS in Cartesian coordinates.
 
 The remapped H(R) drives the oracle; every printed trace must match."""
 import os
@@ -50,11 +52,28 @@ CASES = {
 # topsurface_atom_index = 2 on the three-atom cell: atom 2, not the highest,
 # is shifted to 0.95 along R3'
 CASES["remap_top2_soc_ndim20"] = dict(CASES[NAME], hr="remap_top2_soc_ndim20_hr.dat", top_atom=2)
+# the three-atom cell with the lattice in Bohr and Cartesian positions (Bohr)
+CASES["remap_cart_bohr_soc_ndim20"] = dict(CASES[NAME], hr="remap_cart_bohr_soc_ndim20_hr.dat",
+                                           bohr=True,
+                                           cartesian=[(0.0, 1.0, 0.7), (0.4, 1.0, 0.4),
+                                                      (0.6, 0.2, 0.5)])
 # the field in Tesla (Bmagnitude, Btheta, Bphi, g = 2.5) instead of an energy
 CASES["bhz_zeeman_tesla"] = dict(CASES["bhz_zeeman"], zeeman=dict(bmagnitude=2000.0, btheta=60.0,
                                                                   bphi=30.0, gfactor=2.5))
 CASE = CASES[NAME]
 ANGSTROM2ATOMIC = 1.0 / 0.529177210903
+
+
+def lattice_and_positions(c):
+    """Lattice rows in Bohr and fractional positions: the LATTICE card in
+    Angstrom unless the case says Bohr, positions Direct unless given in
+    Cartesian coordinates of the lattice's unit (src/readinput.f90:772-787,
+    871-933)."""
+    lat = np.array(c["lattice"], float) * (1.0 if c.get("bohr") else ANGSTROM2ATOMIC)
+    if "cartesian" in c:
+        cart = np.array(c["cartesian"], float) * (1.0 if c.get("bohr") else ANGSTROM2ATOMIC)
+        return lat, np.linalg.solve(lat.T, cart.T).T
+    return lat, np.array(c["positions"], float)
 
 
 def remapped(name=NAME):
@@ -68,8 +87,8 @@ def remapped(name=NAME):
     hr *= M.EV2HARTREE
     if "zeeman" in c:
         hr = M.add_zeeman(mdl.irvec, hr, *M.zeeman_field(**c["zeeman"]))
-    lat = np.array(c["lattice"]) * ANGSTROM2ATOMIC
-    return d, C.build_surface_cell(lat, np.array(c["positions"]), c["nprojs"], c["soc"],
+    lat, pos = lattice_and_positions(c)
+    return d, C.build_surface_cell(lat, pos, c["nprojs"], c["soc"],
                                    mdl.irvec, hr, mdl.ndegen, c["surface"], c.get("top_atom", 0))
 
 
@@ -136,7 +155,7 @@ def test_oracle_on_remapped_model_reproduces_reference_files(oracle_lib, name):
     for side, v in (("l", np.log(dl[:, 0])), ("r", np.log(dr[:, 0])), ("bulk", np.log(np.abs(db[:, 0])))):
         ref = read_table(os.path.join(d, f"arc.dat_{side}"))[1]
         assert np.allclose(v, ref[:, 2], rtol=0, atol=2e-8 + 2e-8 * np.abs(ref[:, 2]).max()), side
-        # the k coordinates follow the new cell's surface reciprocal vectors
+        # the k coordinates follow the new cell's surface reciprocal vectors (1/Angstrom)
         shape = (k12[:, :1] * sc.ka2 + k12[:, 1:] * sc.kb2) * ANGSTROM2ATOMIC
         assert np.allclose(shape, ref[:, :2], atol=2e-8)
 
@@ -183,6 +202,7 @@ def read_dump(path):
      True, CASES["bhz_zeeman"]["surface"], 0.0),
     ("remap_top2_soc_ndim20", CASE["lattice"], CASE["positions"], CASE["nprojs"], True,
      CASE["surface"], 0.0),
+    ("remap_cart_bohr_soc_ndim20", None, None, CASE["nprojs"], True, CASE["surface"], 0.0),
     ("soc_ndim30", [[1, 0, 0], [0, 1, 0], [0, 0, 1]], [(0, 0, 0), (0.1, 0.05, 0.2)], [3, 2], True,
      [[1, 0, 0], [0, 1, 0]], 0.0),
     ("ndim16", [[1.1, 0, 0], [0.3, 0.9, 0], [0, 0.1, 1.3]], [(0, 0, 0)], [8], False,
@@ -209,8 +229,11 @@ def test_fortran_host_remap_matches(tmp_path, name, lat, pos, nprojs, soc, surfa
     hr *= M.EV2HARTREE
     if "zeeman" in CASES.get(name, {}):
         hr = M.add_zeeman(mdl.irvec, hr, *M.zeeman_field(**CASES[name]["zeeman"]))
-    want = C.build_surface_cell(np.array(lat, float) * ANGSTROM2ATOMIC, np.array(pos, float), nprojs,
-                                soc, mdl.irvec, hr, mdl.ndegen, surface,
+    if lat is None:
+        lat_bohr, pos = lattice_and_positions(CASES[name])
+    else:
+        lat_bohr, pos = np.array(lat, float) * ANGSTROM2ATOMIC, np.array(pos, float)
+    want = C.build_surface_cell(lat_bohr, pos, nprojs, soc, mdl.irvec, hr, mdl.ndegen, surface,
                                 CASES.get(name, {}).get("top_atom", 0))
     assert np.array_equal(got["irvec"], want.irvec)
     assert np.allclose(got["hr"], want.hr, rtol=1e-15, atol=1e-18)
