@@ -197,6 +197,14 @@ CASES = {
                                   positions=[(0.1, 0.2, 0.3), (0.6, 0.55, 0.75), (0.35, 0.8, 0.1)],
                                   surface="1 1 0\n0 0 1", params="topsurface_atom_index = 2\n",
                                   runs=["base"]),
+    # Particle = 'phonon': H(R) read as eV^2 and the decimation run at omega^2
+    # (src/readHmnR.f90:155-159, src/surfstat.f90:90-96, src/fermiarc.f90:196-200);
+    # the omega grid's negative half prints as |omega| after the square root
+    "phonon_ndim16": dict(mdl=M.model_ndim16, soc=0, nprojs=[8], np_=2, eta=0.002, earc=0.3,
+                          efermi=0.05, nomega=21, nk1=11, nk2=7,
+                          lattice=["1.1 0.0 0.0", "0.3 0.9 0.0", "0.0 0.1 1.3"],
+                          edits=[(r"^(Hrfile = '.*')$", "\\1\nParticle = 'phonon'")],
+                          runs=["base"]),
     # the three-atom cell with its lattice in Bohr and the atoms in Cartesian
     # coordinates (Bohr), src/readinput.f90:772-787, 871-933
     "remap_cart_bohr_soc_ndim20": dict(mdl=lambda: M.synthetic_model(10, 1, 0.35, M.decay_l1,

@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 # weyl3d and bhz_zeeman: the reference's own examples (examples/3DWeyl-model,
 # examples/BHZ-model with its Zeeman field) with their grids cut down
 CASES = ["ndim16", "soc_ndim30", "soc_rot_ndim20", "remap_soc_ndim20", "remap_top2_soc_ndim20",
-         "remap_cart_bohr_soc_ndim20", "weyl3d", "bhz_zeeman", "bhz_zeeman_tesla"]
+         "remap_cart_bohr_soc_ndim20", "phonon_ndim16", "weyl3d", "bhz_zeeman", "bhz_zeeman_tesla"]
 FILES = ["dos.dat_l", "dos.dat_r", "dos.dat_bulk", "arc.dat_l", "arc.dat_r", "arc.dat_bulk",
          "spindos.dat_l", "spindos.dat_r"]
 WT_SURF = os.path.join(ROOT, "wannier_tools_amd", "bin", "wt_surf")

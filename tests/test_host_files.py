@@ -122,3 +122,39 @@ def test_oracle_reproduces_reference_spindos_files(oracle_lib, name):
         got = np.einsum("ij,jkw->kwi", urot, s).reshape(-1, 3)
         scale = np.abs(ref[:, 2:]).max()
         assert np.all(np.abs(got - ref[:, 2:]) <= 2e-7 * np.abs(ref[:, 2:]) + 2e-9 * scale), side
+
+
+def test_oracle_reproduces_reference_phonon_files(oracle_lib):
+    """Particle = 'phonon': H(R) is read as eV^2 and converted with eV2Hartree
+    twice (src/readHmnR.f90:155-159); surfstat and SurfaceDOSkk decimate at
+    omega^2 with the broadening left in Hartree (src/surfstat.f90:84-96,
+    src/fermiarc.f90:196-200), and surfstat prints sqrt(omega^2) = |omega| in
+    the energy column (src/surfstat.f90:260-265)."""
+    name = "phonon_ndim16"
+    d = os.path.join(GOLDEN, f"files_{name}")
+    mdl = M.read_hr(os.path.join(d, f"{name}_hr.dat"))
+    hr = mdl.hr.astype(np.complex128, copy=True)
+    for ir in np.nonzero(np.all(mdl.irvec == 0, axis=1))[0]:
+        hr[ir][np.diag_indices(mdl.num_wann)] -= 0.05
+    hr *= M.EV2HARTREE * M.EV2HARTREE
+    hr /= mdl.ndegen.astype(np.float64)[:, None, None]
+    lo, hi = -1.0 * M.EV2HARTREE, 1.0 * M.EV2HARTREE
+    om = M.omega_grid(lo, hi, 21)
+    eta = (hi - lo) / 21 * 3.0
+    kp = M.kpath_slab(KPATH, 11)
+    orb = np.arange(1, 9, dtype=np.int32)
+    dl, dr, db, _ = oracle_lib.spectral(8, 2, mdl.irvec, hr, kp, om * om, eta, orb, orb)
+    ref = {s: read_table(os.path.join(d, f"dos.dat_{s}"))[1] for s in ("l", "r", "bulk")}
+    assert np.allclose(ref["l"][:, 1], np.tile(np.abs(om) / M.EV2HARTREE, kp.shape[0]),
+                       rtol=1e-7, atol=1e-8)
+    assert np.allclose(np.log(np.abs(dl)).ravel(), ref["l"][:, 2], rtol=2e-7, atol=2e-7)
+    assert np.allclose(np.log(np.abs(dr)).ravel(), ref["r"][:, 2], rtol=2e-7, atol=2e-7)
+    assert np.allclose(np.log(np.abs(db) + M.EPS9).ravel(), ref["bulk"][:, 2], rtol=2e-7, atol=2e-7)
+    k12, nkx, nky = M.kplane_slab((-0.5, -0.5), (1.0, 0.0), (0.0, 1.0), 11, 7)
+    w = (0.3 * M.EV2HARTREE) ** 2
+    dl, dr, db, _ = oracle_lib.spectral(8, 2, mdl.irvec, hr, k12, [w], 0.002 * M.EV2HARTREE, orb,
+                                        orb, divisor=M.REF_PI)
+    for s, v in (("l", np.log(dl[:, 0])), ("r", np.log(dr[:, 0])), ("bulk", np.log(np.abs(db[:, 0])))):
+        ref = read_table(os.path.join(d, f"arc.dat_{s}"))[1]
+        assert ref.shape[0] == nkx * nky
+        assert np.allclose(v, ref[:, 2], rtol=0, atol=2e-8 + 2e-8 * np.abs(ref[:, 2]).max())
