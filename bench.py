@@ -10,7 +10,7 @@ is the Fourier sum plus the batched decimation and traces.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32_refine|auto]
     python bench.py --workload plane288|mixed [--steps K] [--warmup W]
-    python bench.py --summarize-rocprof TRACE_DIR FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
+    python bench.py --summarize-rocprof [plane288] TRACE_DIR FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
     python bench.py --summarize-trace TRACE_DIR KERNEL [KERNEL ...]
 
 Prints one JSON line.  The decimation runs as rounds of two kernels on two
@@ -56,6 +56,15 @@ from wannier_tools_amd import model as M  # noqa: E402
 
 TRAFFIC_JSON = "decimation_fp64_ndim72_512x512.traffic.json"
 DECIMATION_KERNELS = ("gj_round_kernel", "gemm_round_kernel")
+LARGE_KERNELS = ("gj_large_round_kernel", "gemm_large_round_kernel")
+# per workload: traffic file under profiles/, the stage's kernels, what was run
+TRAFFIC = {
+    "grid72": (TRAFFIC_JSON, DECIMATION_KERNELS, "decimation rounds (gj_round_kernel + "
+               "gemm_round_kernel)", "ndim72 512x512 eta=1e-3 eV"),
+    "plane288": ("decimation_fp64_ndim288_kplane65.traffic.json", LARGE_KERNELS,
+                 "decimation rounds (gj_large_round_kernel + gemm_large_round_kernel)",
+                 "ndim288 65x65 k-plane omega=0 eta=5e-4 eV"),
+}
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
@@ -221,20 +230,24 @@ def summarize_trace(trace, *kernels) -> int:
     return 0
 
 
-def summarize_rocprof(trace, fetch, write, cfetch, cwrite) -> int:
+def summarize_rocprof(*argv) -> int:
     """profiles/ summary of the decimation stage from rocprofv3 runs of
-    `bench.py --steps 1 --warmup 0 --no-cpu-baseline --precision fp64`."""
-    span_ns = _trace_table(trace, DECIMATION_KERNELS)
+    `bench.py --steps 1 --warmup 0 --no-cpu-baseline --precision fp64` (or of
+    `--workload plane288` when the first argument names it): arguments
+    [WORKLOAD] TRACE FETCH WRITE COPY_FETCH COPY_WRITE."""
+    workload = argv[0] if len(argv) == 6 else "grid72"
+    trace, fetch, write, cfetch, cwrite = argv[-5:]
+    fname, kernels, stage, what = TRAFFIC[workload]
+    span_ns = _trace_table(trace, kernels)
     # counter scale: KiB reported for a copy of known size
     copy_bytes = 1 << 30
     f_scale = copy_bytes / (_counter_kib(cfetch, "FETCH_SIZE", ("copyBuffer",)) * 1024)
     w_scale = copy_bytes / (_counter_kib(cwrite, "WRITE_SIZE", ("copyBuffer",)) * 1024)
-    fetch_b = _counter_kib(fetch, "FETCH_SIZE", DECIMATION_KERNELS) * 1024 * f_scale
-    write_b = _counter_kib(write, "WRITE_SIZE", DECIMATION_KERNELS) * 1024 * w_scale
-    json.dump({"stage": "decimation rounds (gj_round_kernel + gemm_round_kernel)",
-               "workload": "ndim72 512x512 eta=1e-3 eV", "fetch_bytes": fetch_b,
+    fetch_b = _counter_kib(fetch, "FETCH_SIZE", kernels) * 1024 * f_scale
+    write_b = _counter_kib(write, "WRITE_SIZE", kernels) * 1024 * w_scale
+    json.dump({"stage": stage, "workload": what, "fetch_bytes": fetch_b,
                "write_bytes": write_b, "fetch_scale": f_scale, "write_scale": w_scale,
-               "span_ns": span_ns}, open(os.path.join(ROOT, "profiles", TRAFFIC_JSON), "w"),
+               "span_ns": span_ns}, open(os.path.join(ROOT, "profiles", fname), "w"),
               indent=1)
     print(f"\nHBM traffic of the stage: fetch {fetch_b:.4e} B (scale {f_scale:.3f}), "
           f"write {write_b:.4e} B (scale {w_scale:.3f}), {(fetch_b + write_b) / span_ns:.1f} GB/s "
@@ -356,6 +369,13 @@ def run_plane288(args, SurfaceGF) -> dict:
     gf.close()
     log(f"[bench] plane288: {ninst / s:.1f} solves/s, decimation {dec:.1f} ms")
     achieved = flops / (dec * 1e-3) / 1e12
+    # HBM bytes of the stage on this workload, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE runs (scaled by a 1 GiB copy calibration)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", TRAFFIC["plane288"][0])
+    if nk == 65 and os.path.exists(tpath):
+        t = json.load(open(tpath))
+        traffic = t["fetch_bytes"] + t["write_bytes"]
     out = {
         "metric": "G(k,w) solves/s @Ndim=288, k-plane at omega=0 (FP64)",
         "value": ninst / s, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -368,7 +388,7 @@ def run_plane288(args, SurfaceGF) -> dict:
                    "stop_rule": "sum|alpha|<=1e-16 Ha, itermax 100", "path": "fp64",
                    "mean_iters": float(its.mean()), "not_converged": int(np.count_nonzero(st & 2))},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": "decimation stage: gj_large_round_kernel<18, double> + "
                                "gemm_large_round_kernel<18, double> rounds on two streams",
                      "flops_per_launch": flops},
@@ -456,7 +476,7 @@ def run_mixed(args, SurfaceGF) -> dict:
 
 
 def main() -> int:
-    if len(sys.argv) == 7 and sys.argv[1] == "--summarize-rocprof":
+    if len(sys.argv) in (7, 8) and sys.argv[1] == "--summarize-rocprof":
         return summarize_rocprof(*sys.argv[2:])
     if len(sys.argv) >= 4 and sys.argv[1] == "--summarize-trace":
         return summarize_trace(*sys.argv[2:])
