@@ -227,7 +227,7 @@ EXAMPLES = {
     # and (0 0 1), R3' found by the reference's search; one KPATH_SLAB line
     # -X -> X; Nk1 = Nk2 = 201 and OmegaNum = 400 cut to 21, 21 and 40
     "weyl3d": dict(src="examples/3DWeyl-model", wtin="wt.in-surfacestates", hr="Weyl3D_hr.dat",
-                   sizes={"OmegaNum": 40, "Nk1": 21, "Nk2": 21}),
+                   sizes={"OmegaNum": 40, "Nk1": 21, "Nk2": 21}, runs=["qpi"]),
     # four-band BHZ model with a 0.5 eV Zeeman field along z (Add_Zeeman_Field,
     # Bx, By, Bz on one line): SURFACE given as three rows (0 0 1), (1 0 0),
     # (0 1 0), three KPATH_SLAB lines, NP and Eta_Arc left at their defaults.
@@ -312,6 +312,17 @@ def main() -> None:
             for fn in ["wt.in", hr] + [fn for fn in OUTPUTS if os.path.exists(os.path.join(d, fn))]:
                 shutil.copy(os.path.join(d, fn), os.path.join(dst, fn))
         print(f"wrote {dst}")
+        # further driver runs on the same example input, each in a directory of its own
+        for run in c.get("runs", []):
+            control, outputs = RUNS[run]
+            sub = os.path.join(dst, run)
+            with tempfile.TemporaryDirectory() as d:
+                hr = write_example(d, dict(c, control=control))
+                run_wtx(d, outputs[0])
+                os.makedirs(sub)
+                for fn in ["wt.in", hr] + [fn for fn in outputs if os.path.exists(os.path.join(d, fn))]:
+                    shutil.copy(os.path.join(d, fn), os.path.join(sub, fn))
+            print(f"wrote {sub}")
 
 
 if __name__ == "__main__":
