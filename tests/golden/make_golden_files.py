@@ -227,7 +227,7 @@ EXAMPLES = {
     # and (0 0 1), R3' found by the reference's search; one KPATH_SLAB line
     # -X -> X; Nk1 = Nk2 = 201 and OmegaNum = 400 cut to 21, 21 and 40
     "weyl3d": dict(src="examples/3DWeyl-model", wtin="wt.in-surfacestates", hr="Weyl3D_hr.dat",
-                   sizes={"OmegaNum": 40, "Nk1": 21, "Nk2": 21}, runs=["qpi"]),
+                   sizes={"OmegaNum": 40, "Nk1": 21, "Nk2": 21}, runs=["qpi", "qpi_kpath", "spintexture"]),
     # four-band BHZ model with a 0.5 eV Zeeman field along z (Add_Zeeman_Field,
     # Bx, By, Bz on one line): SURFACE given as three rows (0 0 1), (1 0 0),
     # (0 1 0), three KPATH_SLAB lines, NP and Eta_Arc left at their defaults.

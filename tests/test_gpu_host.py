@@ -32,7 +32,8 @@ WT_GPU = os.path.join(ROOT, "oracle", "_ref", "wt_gpu.x")
 # SurfaceDOSkk's QPI and spin texture drivers and surfstat_jdos, each run in its
 # own directory
 RUNS = [("ndim16", "qpi"), ("qpi_soc_ndim20", "qpi"), ("qpi_soc_ndim20", "spintexture"),
-        ("ndim16", "qpi_kpath"), ("soc_rot_ndim20", "qpi_kpath"), ("weyl3d", "qpi")]
+        ("ndim16", "qpi_kpath"), ("soc_rot_ndim20", "qpi_kpath"), ("weyl3d", "qpi"),
+        ("weyl3d", "qpi_kpath"), ("weyl3d", "spintexture")]
 
 
 def run_case(exe, name, tmp_path, run=None):
