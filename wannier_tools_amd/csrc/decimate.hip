@@ -1412,12 +1412,13 @@ __global__ void __launch_bounds__(kBlock) gemm_large_round_kernel(DecimateArgs a
     }
 }
 
-constexpr int kMaxQ = 18;
+constexpr int kMaxQ = 20;
 constexpr int kRoundsPerBatch = 8;
 
 // Orders are padded to 16 Q (the pad rows and columns are exact zeros);
-// kernels exist for Q = 1..5 (matrix in LDS) and Q = 6..18 (matrix in the
-// slot's global workspace).
+// kernels exist for Q = 1..5 (matrix in LDS) and Q = 6..20 (matrix in the
+// slot's global workspace; 20 is the most rows the panel wave holds, five per
+// lane).
 int padded_q(int n) { return (n + 15) / 16; }
 int decimate_max_q() { return kMaxQ; }
 
@@ -1446,7 +1447,7 @@ static int launch_round(const DecimateArgs &a, int per_set, int set, bool gj, hi
 }
 
 #define WTSURF_FOR_EACH_Q(X) \
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18)
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
 
 template <typename R>
 static int dispatch_round(const DecimateArgs &a, int q, int per_set, int set, bool gj, hipStream_t s) {
