@@ -148,14 +148,15 @@ def test_uneven_orders_vs_oracle(gpu, oracle_lib, nw, np_):
 
 
 @pytest.mark.parametrize("nw,np_", [(48, 2), (50, 2), (36, 3), (65, 2), (72, 3), (144, 2),
-                                     (152, 2), (160, 2)])
+                                     (152, 2), (160, 2), (168, 2), (192, 2)])
 def test_large_orders_vs_oracle(gpu, oracle_lib, nw, np_):
     """Orders beyond 80, where the Gauss-Jordan matrix lives in global memory
     and the GEMMs run in blocks: Ndim 96 (Q 6), 100 (Q 7, 12 pad rows), 108
     (Q 7), 130 (Q 9, 14 pad rows), 216 (Q 14; the largest order of the
     mixed-model set), 288 (Q 18; the 144-orbital spinful k-plane model), 304
-    (Q 19) and 320 (Q 20, the largest order built), with custom orbital lists
-    and the trace divisor pi."""
+    (Q 19), 320 (Q 20, five full row sets in the panel), 336 (Q 21, the first
+    with six) and 384 (Q 24, the largest order built), with custom orbital
+    lists and the trace divisor pi."""
     _say("large orders", nw, np_)
     mdl = M.synthetic_model(nw, 2, 0.5, M.decay_l2(1.5), seed=200 + nw)
     irvec, hr = mdl.surface_cell()

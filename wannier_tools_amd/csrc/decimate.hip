@@ -417,7 +417,7 @@ __device__ __forceinline__ bool gj_inverse(unsigned char *sbuf, unsigned char *t
     constexpr int NP = 16 * Q;
     constexpr int LD = NP + 1;
     constexpr int NWAVES = NT / kWave;
-    static_assert(NP <= 320, "the panel wave holds at most five rows per lane");
+    static_assert(NP <= 384, "the panel wave holds at most six rows per lane");
     static_assert(NWAVES >= 2, "the lookahead needs a panel wave and update waves");
     const int tid = threadIdx.x;
     const int wave = uniform(tid >> 6), lane = tid & 63;
@@ -1412,13 +1412,14 @@ __global__ void __launch_bounds__(kBlock) gemm_large_round_kernel(DecimateArgs a
     }
 }
 
-constexpr int kMaxQ = 20;
+constexpr int kMaxQ = 24;
 constexpr int kRoundsPerBatch = 8;
 
 // Orders are padded to 16 Q (the pad rows and columns are exact zeros);
-// kernels exist for Q = 1..5 (matrix in LDS) and Q = 6..20 (matrix in the
-// slot's global workspace; 20 is the most rows the panel wave holds, five per
-// lane).
+// kernels exist for Q = 1..5 (matrix in LDS) and Q = 6..24 (matrix in the
+// slot's global workspace).  Q = 24 is the limit: the panel wave then holds
+// six rows per lane (256 VGPRs), and the Gauss-Jordan kernel's LDS comes to
+// 162,864 of the CU's 163,840 bytes.
 int padded_q(int n) { return (n + 15) / 16; }
 int decimate_max_q() { return kMaxQ; }
 
@@ -1447,7 +1448,8 @@ static int launch_round(const DecimateArgs &a, int per_set, int set, bool gj, hi
 }
 
 #define WTSURF_FOR_EACH_Q(X) \
-    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
+    X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20) \
+        X(21) X(22) X(23) X(24)
 
 template <typename R>
 static int dispatch_round(const DecimateArgs &a, int q, int per_set, int set, bool gj, hipStream_t s) {
