@@ -91,8 +91,11 @@ MIXED_MODELS = ((8, 1, 40), (16, 2, 41), (36, 3, 42), (72, 3, 43))  # (Num_wann,
 MIXED_PER_MODEL = 25000
 
 
-def workload_plane288(nk: int):
-    mdl = M.model_ndim288()
+def workload_plane288(nk: int, nw: int = 144):
+    # nw = 144 is the named model; other sizes use the same recipe (R in
+    # [-3, 3]^3, spinful, seed 5) to time the kernels at other orders
+    mdl = M.model_ndim288() if nw == 144 else M.synthetic_model(
+        nw, 3, 0.5, M.decay_l2(1.5), seed=5, soc_sigma=0.05, name=f"nw{nw}_soc_r3")
     irvec, hr = mdl.surface_cell()
     kpts, _, _ = M.kplane_slab((-0.5, -0.5), (1.0, 0.0), (0.0, 1.0), nk, nk)
     return irvec, hr, kpts, np.array([0.0]), PLANE_ETA_EV * M.EV2HARTREE
@@ -355,11 +358,13 @@ def run_grid72(args, SurfaceGF) -> dict:
 def run_plane288(args, SurfaceGF) -> dict:
     """The Ndim = 288 model on a k-plane at omega = 0 (FP64)."""
     nk = args.nk or 65
-    irvec, hr, kpts, omegas, eta = workload_plane288(nk)
+    nw = args.plane_nw
+    irvec, hr, kpts, omegas, eta = workload_plane288(nk, nw)
     ninst = kpts.shape[0]
-    n = PLANE_NW * PLANE_NP
+    n = nw * PLANE_NP
+    q = (n + 15) // 16
     gf = SurfaceGF(0)
-    gf.set_model_arrays(PLANE_NW, PLANE_NP, irvec, hr)
+    gf.set_model_arrays(nw, PLANE_NP, irvec, hr)
     gf.upload_grid(kpts, omegas)
     gf.eta = eta
     log(f"[bench] plane288: {ninst} instances, Ndim={n}")
@@ -373,30 +378,30 @@ def run_plane288(args, SurfaceGF) -> dict:
     # FETCH_SIZE / WRITE_SIZE runs (scaled by a 1 GiB copy calibration)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", TRAFFIC["plane288"][0])
-    if nk == 65 and os.path.exists(tpath):
+    if nk == 65 and nw == PLANE_NW and os.path.exists(tpath):
         t = json.load(open(tpath))
         traffic = t["fetch_bytes"] + t["write_bytes"]
     out = {
-        "metric": "G(k,w) solves/s @Ndim=288, k-plane at omega=0 (FP64)",
+        "metric": f"G(k,w) solves/s @Ndim={n}, k-plane at omega=0 (FP64)",
         "value": ninst / s, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": s * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
         "config": {"workload": f"surface GF k-plane {nk}x{nk} over [-0.5,0.5]^2 at omega=0, "
-                               "144-orbital spinful synthetic model, Np=2",
-                   "ndim": n, "num_wann": PLANE_NW, "np": PLANE_NP, "nr": int(irvec.shape[0]),
+                               f"{nw}-orbital spinful synthetic model, Np=2",
+                   "ndim": n, "num_wann": nw, "np": PLANE_NP, "nr": int(irvec.shape[0]),
                    "nk": ninst, "instances": ninst, "eta_ev": PLANE_ETA_EV, "divisor": "pi",
                    "stop_rule": "sum|alpha|<=1e-16 Ha, itermax 100", "path": "fp64",
                    "mean_iters": float(its.mean()), "not_converged": int(np.count_nonzero(st & 2))},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": "decimation stage: gj_large_round_kernel<18, double> + "
-                               "gemm_large_round_kernel<18, double> rounds on two streams",
+                     "kernel": f"decimation stage: gj_large_round_kernel<{q}, double> + "
+                               f"gemm_large_round_kernel<{q}, double> rounds on two streams",
                      "flops_per_launch": flops},
     }
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline ...")
         total = (args.cpu_per_worker or 6) * args.cpu_workers
-        out["cpu_baseline"] = cpu_baseline([(PLANE_NW, PLANE_NP, irvec, hr)],
+        out["cpu_baseline"] = cpu_baseline([(nw, PLANE_NP, irvec, hr)],
                                            grid_jobs(kpts, omegas, eta, total), args.cpu_workers,
                                            f"k points strided over the {nk}x{nk} plane")
     return out
@@ -495,6 +500,8 @@ def main() -> int:
                     help="CPU baseline solves per worker (default 400 grid72, 6 plane288, "
                          "6 per model mixed)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--plane-nw", type=int, default=PLANE_NW,
+                    help="plane288 orbitals (default 144, Ndim 288; up to 192, Ndim 384)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
