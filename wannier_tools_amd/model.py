@@ -247,18 +247,19 @@ def read_hr(path: str, e_fermi: float = 0.0) -> TBModel:
     if data.shape[0] != nr * nw * nw:
         raise ValueError(f"{path}: expected {nr * nw * nw} H(R) lines, found {data.shape[0]}")
     rows = np.rint(data[:, 0:5]).astype(np.int64)
-    first: dict[tuple[int, int, int], int] = {}
-    for r in map(tuple, rows[:, 0:3]):
-        first.setdefault(r, len(first))
-    if len(first) != nr:
-        raise ValueError(f"{path}: {len(first)} distinct R vectors, header says {nr}")
+    # R vectors numbered in order of first appearance
+    uniq, first, inverse = np.unique(rows[:, 0:3], axis=0, return_index=True, return_inverse=True)
+    if uniq.shape[0] != nr:
+        raise ValueError(f"{path}: {uniq.shape[0]} distinct R vectors, header says {nr}")
+    order = np.argsort(first)
+    rank = np.empty(nr, dtype=np.int64)
+    rank[order] = np.arange(nr)
     m, n = rows[:, 3] - 1, rows[:, 4] - 1
     if m.min() < 0 or n.min() < 0 or m.max() >= nw or n.max() >= nw:
         raise ValueError(f"{path}: orbital index outside 1..{nw}")
-    ir = np.array([first[r] for r in map(tuple, rows[:, 0:3])], dtype=np.int64)
     hr = np.zeros((nr, nw, nw), dtype=np.complex128)
-    hr[ir, m, n] = data[:, 5] + 1j * data[:, 6]
-    irvec = np.array(list(first), dtype=np.int32).reshape(nr, 3)
+    hr[rank[inverse.ravel()], m, n] = data[:, 5] + 1j * data[:, 6]
+    irvec = uniq[order].astype(np.int32)
     return TBModel(num_wann=nw, irvec=irvec, hr=hr,
                    ndegen=np.array(nd[:nr], dtype=np.int32), e_fermi=e_fermi)
 
