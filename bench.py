@@ -10,7 +10,7 @@ is the Fourier sum plus the batched decimation and traces.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32_refine|auto]
     python bench.py --workload plane288|mixed [--steps K] [--warmup W]
-    python bench.py --summarize-rocprof [plane288] TRACE_DIR FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
+    python bench.py --summarize-rocprof [plane288|plane384] TRACE_DIR FETCH_DIR WRITE_DIR COPY_FETCH_DIR COPY_WRITE_DIR
     python bench.py --summarize-trace TRACE_DIR KERNEL [KERNEL ...]
 
 Prints one JSON line.  The decimation runs as rounds of two kernels on two
@@ -64,6 +64,9 @@ TRAFFIC = {
     "plane288": ("decimation_fp64_ndim288_kplane65.traffic.json", LARGE_KERNELS,
                  "decimation rounds (gj_large_round_kernel + gemm_large_round_kernel)",
                  "ndim288 65x65 k-plane omega=0 eta=5e-4 eV"),
+    "plane384": ("decimation_fp64_ndim384_kplane45.traffic.json", LARGE_KERNELS,
+                 "decimation rounds (gj_large_round_kernel + gemm_large_round_kernel)",
+                 "ndim384 45x45 k-plane omega=0 eta=5e-4 eV (--plane-nw 192 --nk 45)"),
 }
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
@@ -377,8 +380,9 @@ def run_plane288(args, SurfaceGF) -> dict:
     # HBM bytes of the stage on this workload, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE runs (scaled by a 1 GiB copy calibration)
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", TRAFFIC["plane288"][0])
-    if nk == 65 and nw == PLANE_NW and os.path.exists(tpath):
+    key = {(65, PLANE_NW): "plane288", (45, 192): "plane384"}.get((nk, nw))
+    tpath = os.path.join(ROOT, "profiles", TRAFFIC[key][0]) if key else ""
+    if key and os.path.exists(tpath):
         t = json.load(open(tpath))
         traffic = t["fetch_bytes"] + t["write_bytes"]
     out = {
